@@ -1,0 +1,188 @@
+/*
+ * mpsx.h - C ABI of the MI355X (gfx950) MPS gate-application library.
+ *
+ * The reference (grmlarose/mpsim) is pure Python with no FFI; its hot path
+ * lives behind the Python object API of mpsim.MPS. Each entry point below
+ * replaces one arithmetic step of that API; the Python package mpsim_amd
+ * keeps the reference's names, arguments and exceptions and calls these
+ * through ctypes. Citations are to the reference's files.
+ *
+ * Conventions
+ *   - All tensors are complex64 (interleaved float re, im) in device memory,
+ *     owned by the caller. A site tensor is [chi_l, d, chi_r] row-major;
+ *     chain ends carry a dummy bond of size 1.
+ *   - Every batched entry point takes the same descriptor array twice:
+ *     `h` in host memory (read by the library to size launches) and `d` in
+ *     device memory (read by the kernels). The two must hold identical
+ *     bytes and `d` must stay valid until the enqueued work has finished.
+ *   - `stream` is a hipStream_t (NULL = the default stream). Calls enqueue
+ *     work and return; they never synchronise the stream unless stated.
+ *   - The library allocates no device memory. Scratch space is passed in by
+ *     the caller (`ws`, `ws_bytes`), sized by the matching *_workspace call.
+ *   - Return value: 0 on success, otherwise an MPSX_E* code; the message is
+ *     available from mpsx_last_error() on the calling thread.
+ */
+#ifndef MPSX_H
+#define MPSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPSX_OK 0
+#define MPSX_EINVAL 1      /* bad argument (shape, size, null pointer)      */
+#define MPSX_EHIP 2        /* a HIP runtime call failed                     */
+#define MPSX_EWORKSPACE 3  /* workspace too small                           */
+#define MPSX_ENOTCONV 4    /* SVD hit the sweep limit before converging     */
+
+typedef void *mpsx_stream; /* hipStream_t */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int mpsx_version(void);
+
+/* Message for the last failed call on this thread ("" if none). */
+const char *mpsx_last_error(void);
+
+/* gfx arch name of `device` (e.g. "gfx950") and its compute-unit count. */
+int mpsx_device_info(int device, char *arch, int arch_len, int *num_cus);
+
+/* sizeof of the ABI structs, so bindings can check their layouts:
+ * 0 mpsx_site_op, 1 mpsx_two_site_op, 2 mpsx_gemm_op, 3 mpsx_scale_op,
+ * 4 mpsx_svd_opts, 5 mpsx_plan. Returns 0 for an unknown index. */
+size_t mpsx_sizeof(int which);
+
+/* ------------------------------------------------------------------------
+ * One-site gate.
+ * Replaces MPS.apply_one_qudit_gate (mpsim/core.py:753-826): gate edge 1
+ * contracts the site's physical leg, gate edge 0 becomes the new leg,
+ *     out[l, o, r] = sum_p G[o, p] * a[l, p, r].
+ * `out` may alias `a`.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    const void *a;    /* [chi_l, d, chi_r]                       */
+    void *out;        /* [chi_l, d, chi_r]                       */
+    const void *gate; /* [d, d] row-major G[o, p]                */
+    int64_t chi_l;
+    int64_t chi_r;
+    int32_t d;
+    int32_t reserved;
+} mpsx_site_op;
+
+int mpsx_apply_one_site(mpsx_stream stream, const mpsx_site_op *h,
+                        const mpsx_site_op *d, int count);
+
+/* ------------------------------------------------------------------------
+ * Two-site update on adjacent sites (i, i+1).
+ * Replaces the arithmetic of MPS.apply_two_qudit_gate
+ * (mpsim/core.py:1045-1152) and TensorNetwork 0.2.1's split_node_full_svd
+ * (called at core.py:1132-1137):
+ *   theta[l, o1, o2, r] = sum G[o1, o2, p1, p2] a[l, p1, m] b[m, p2, r]
+ *   theta (rows (l,o1), cols (o2,r)) = U S V^H, singular values descending,
+ *   ties in input order, truncated to k columns (zero values are kept).
+ *   absorb right (flags bit0 = 0, keep_left_canonical=True):
+ *       a_out = U[:, :k]            as [chi_l, d, k]
+ *       b_out = S[:k] V^H[:k, :]    as [k, d, chi_r]
+ *   absorb left  (flags bit0 = 1, keep_left_canonical=False):
+ *       a_out = U[:, :k] S[:k],  b_out = V^H[:k, :]
+ * The caller computes k = min(d*chi_l, d*chi_r, maxsvals) (core.py:1110-1130).
+ * The SVD is a blocked one-sided Jacobi on the side that is not scaled by S,
+ * so the scaled side never divides by a singular value.
+ * ---------------------------------------------------------------------- */
+/* Launch plan of one problem. Filled by mpsx_two_site_plan; callers copy it
+ * to the device descriptor unchanged and never edit it. */
+typedef struct {
+    int64_t ws_off;    /* byte offset of this problem's scratch in ws     */
+    int64_t tile_base; /* exclusive prefix of theta tiles                 */
+    int64_t pair_base; /* exclusive prefix of Jacobi pivot pairs          */
+    int64_t row_base;  /* exclusive prefix of padded rows                 */
+    int64_t out_base;  /* exclusive prefix of output rows (d*chi_l + k)   */
+    int32_t C;         /* rows orthogonalised (d*chi_l or d*chi_r)         */
+    int32_t R;         /* row length (the other dimension)                */
+    int32_t Cp;        /* C padded to nb*b                                */
+    int32_t LD;        /* row stride of the working matrix (elements)     */
+    int32_t nb;        /* number of row blocks (even)                     */
+    int32_t b;         /* rows per block                                  */
+} mpsx_plan;
+
+typedef struct {
+    const void *a;     /* [chi_l, d, chi_m]                              */
+    const void *b;     /* [chi_m, d, chi_r]                              */
+    const void *gate;  /* [d*d, d*d] row-major G[(o1,o2), (p1,p2)]       */
+    void *a_out;       /* [chi_l, d, k]                                  */
+    void *b_out;       /* [k, d, chi_r]                                  */
+    float *s_out;      /* k singular values, or NULL                     */
+    double *discarded; /* sum of squared discarded values, or NULL       */
+    int32_t chi_l, chi_m, chi_r, d;
+    int32_t k;
+    int32_t flags;     /* bit0: absorb S into the left factor            */
+    mpsx_plan plan;    /* library-owned, see mpsx_two_site_plan          */
+} mpsx_two_site_op;
+
+typedef struct {
+    int32_t block;      /* Jacobi block width b (pivot is 2b x 2b); 0 = auto  */
+    int32_t max_sweeps; /* outer sweep limit; 0 = default (30)              */
+    int32_t check_every;/* host convergence poll period in sweeps; 0 = never
+                           poll (graph-capture safe: runs max_sweeps with
+                           device-side early exit)                         */
+    int32_t inner_sweeps;/* pivot eigensolver sweep limit; 0 = default     */
+    double tol;         /* relative orthogonality tolerance; 0 = default   */
+} mpsx_svd_opts;
+
+/* Validates the batch, fills every h[i].plan and returns the workspace bytes
+ * mpsx_two_site needs. Call it before copying h to the device. */
+int mpsx_two_site_plan(mpsx_two_site_op *h, int count,
+                       const mpsx_svd_opts *opts, size_t *bytes);
+
+/* Batched two-site update. Problems are independent (disjoint site pairs,
+ * possibly of different circuits). `sweeps_out` (host, may be NULL)
+ * receives the number of outer sweeps launched. */
+int mpsx_two_site(mpsx_stream stream, const mpsx_two_site_op *h,
+                  const mpsx_two_site_op *d, int count,
+                  const mpsx_svd_opts *opts, void *ws, size_t ws_bytes,
+                  int *sweeps_out);
+
+/* ------------------------------------------------------------------------
+ * Grouped complex GEMM used by the read-out contractions:
+ *   C = alpha * op(A) * op(B) (+ C if accumulate),  op = identity, or
+ *   conjugate-transpose when the corresponding flag bit is set.
+ * Replaces the tn.contract_between chains of MPS.wavefunction
+ * (core.py:483-500) and MPS.inner_product (core.py:507-561).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    const void *A;
+    const void *B;
+    void *C;
+    int64_t m, n, k;
+    int64_t lda, ldb, ldc; /* row strides (elements) of the stored matrices */
+    float alpha_re, alpha_im;
+    int32_t flags;         /* bit0: op(A)=A^H, bit1: op(B)=B^H, bit2: C += */
+    int32_t reserved;
+} mpsx_gemm_op;
+
+int mpsx_gemm(mpsx_stream stream, const mpsx_gemm_op *h, const mpsx_gemm_op *d,
+              int count);
+
+/* ------------------------------------------------------------------------
+ * Scale every element of each buffer by a complex factor, optionally
+ * conjugating. Replaces the tensor rescaling of MPS.renormalize
+ * (core.py:591-594) and MPS.dagger (core.py:502-505).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    void *x;
+    int64_t n;          /* complex elements */
+    float re, im;
+    int32_t conj;
+    int32_t reserved;
+} mpsx_scale_op;
+
+int mpsx_scale(mpsx_stream stream, const mpsx_scale_op *h,
+               const mpsx_scale_op *d, int count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPSX_H */

@@ -1,0 +1,505 @@
+"""MPS state and operations on device-resident complex64 tensors.
+
+Drop-in counterpart of mpsim.MPS / mpsim.MPSOperation (mpsim/core.py) for
+the gate-application path: same names, keyword arguments and exceptions.
+Site i is a torch complex64 tensor [chi_l, d, chi_r] on the GPU (ends carry
+a dummy bond of size 1); all arithmetic runs in libmpsx's HIP kernels.
+
+Known, documented differences from the reference:
+  - Site tensors are [chi_l, d, chi_r] (the reference's TensorNetwork axis
+    order is internal and unobservable through the gate path).
+  - Storage stays complex64; the reference promotes tensors to complex128
+    after float64/complex128 gates (gates.py:195-212, 272-285).
+  - The per-gate norm the reference appends to `_norms` after every
+    two-qudit gate (core.py:1160-1161, "only for convenience in
+    benchmarking") is off by default; MPS(track_norms=True) turns it on.
+  - A non-unitary one-qudit gate is applied and the state renormalised to
+    its previous norm (core.py:814-845), but the bond-shrinking
+    orthonormalisation SVD (core.py:847-939) is not performed.
+  - `maxsvals` < 0 raises ValueError (the reference slices with it).
+"""
+from __future__ import annotations
+
+from copy import deepcopy
+from typing import List, Optional, Sequence, Tuple, Union
+
+import numpy as np
+
+from . import _native as nat
+from .engine import Engine, TwoSiteItem
+from .gates import (Node, as_tensor, cnot, haar_random_unitary, hgate, is_unitary, rgate,
+                    swap, xgate)
+
+
+class CannotConvertToMPSOperation(Exception):
+    pass
+
+
+class MPSOperation:
+    """A gate and the qudit indices it acts on (core.py:21-156)."""
+
+    def __init__(self, node, qudit_indices: Union[int, Tuple[int, ...]], qudit_dimension: int = 2):
+        self._node = node if isinstance(node, Node) or hasattr(node, "tensor") else Node(node)
+        if isinstance(qudit_indices, (int, np.integer)):
+            qudit_indices = (int(qudit_indices),)
+        self._qudit_indices = tuple(int(q) for q in qudit_indices)
+        self._qudit_dimension = int(qudit_dimension)
+
+    @property
+    def qudit_indices(self) -> Tuple[int, ...]:
+        return self._qudit_indices
+
+    @property
+    def qudit_dimension(self) -> int:
+        return self._qudit_dimension
+
+    @property
+    def num_qudits(self) -> int:
+        return len(self._qudit_indices)
+
+    def node(self, copy: bool = True):
+        """core.py:86-95: a copy by default."""
+        if not copy:
+            return self._node
+        return Node(np.array(as_tensor(self._node), copy=True), getattr(self._node, "name", None))
+
+    def tensor(self, reshape_to_square_matrix: bool = True) -> np.ndarray:
+        t = deepcopy(as_tensor(self._node))
+        if reshape_to_square_matrix:
+            dim = self._qudit_dimension ** self.num_qudits
+            t = np.reshape(t, (dim, dim))
+        return t
+
+    def is_valid(self) -> bool:
+        """core.py:113-129: shape (d,)*2k with all edges dangling."""
+        d = self._qudit_dimension
+        t = as_tensor(self._node)
+        if t.shape != tuple([d] * 2 * self.num_qudits):
+            return False
+        nondangling = getattr(self._node, "has_nondangling_edge", None)
+        if callable(nondangling) and nondangling():
+            return False
+        return True
+
+    def is_unitary(self) -> bool:
+        return is_unitary(self.tensor(reshape_to_square_matrix=True))
+
+    def is_hermitian(self) -> bool:
+        from .gates import is_hermitian
+        return is_hermitian(self.tensor(reshape_to_square_matrix=True))
+
+    def is_single_qudit_operation(self) -> bool:
+        return self.num_qudits == 1
+
+    def is_two_qudit_operation(self) -> bool:
+        return self.num_qudits == 2
+
+    def __str__(self) -> str:
+        return f"Tensor {getattr(self._node, 'name', '')} on qudit(s) {self._qudit_indices}."
+
+
+def max_bond_dimensions_for(n: int, d: int) -> List[int]:
+    """core.py:235-242."""
+    half = [d ** (i + 1) for i in range(n // 2)]
+    out = half + list(reversed(half))
+    if n % 2 == 0:
+        out.remove(d ** (n // 2))
+    return out
+
+
+def parse_split_options(kwargs, max_bond_of_left: int) -> Tuple[Optional[int], bool]:
+    """core.py:1104-1130: (maxsvals or None, keep_left_canonical)."""
+    keep_left = kwargs["keep_left_canonical"] if "keep_left_canonical" in kwargs else True
+    if "fraction" in kwargs and "maxsvals" in kwargs:
+        raise ValueError("Only one of (fraction, maxsvals) can be provided as kwargs.")
+    maxsvals = None
+    if "fraction" in kwargs:
+        fraction = kwargs.get("fraction")
+        if not (0 <= fraction <= 1):
+            raise ValueError("Keyword fraction must be between 0 and 1 but is", fraction)
+        maxsvals = int(round(fraction * max_bond_of_left))
+    if "maxsvals" in kwargs:
+        maxsvals = int(kwargs.get("maxsvals"))
+    if maxsvals is not None and maxsvals < 0:
+        raise ValueError(f"maxsvals must be non-negative but is {maxsvals}")
+    return maxsvals, bool(keep_left)
+
+
+class MPS:
+    """Matrix product state on one GPU (core.py:159-1423, gate path)."""
+
+    def __init__(self, nqudits: int, qudit_dimension: int = 2, tensor_prefix: str = "q", *,
+                 device=None, track_norms: bool = False, _empty: bool = False) -> None:
+        if nqudits < 2:
+            raise ValueError(f"Number of qudits must be greater than 2 but is {nqudits}.")
+        self._nqudits = int(nqudits)
+        self._qudit_dimension = int(qudit_dimension)
+        self._prefix = tensor_prefix
+        self._engine = Engine.get(device)
+        self._track_norms = track_norms
+        self._norms: List[float] = []
+        self._max_bond_dimensions = max_bond_dimensions_for(self._nqudits, self._qudit_dimension)
+        if _empty:
+            self._tensors = []
+            return
+        torch = self._engine.torch
+        # core.py:190-218: every site |0>, complex64.
+        block = torch.zeros((self._nqudits, 1, self._qudit_dimension, 1), dtype=torch.complex64,
+                            device=self._engine.dev)
+        block[:, 0, 0, 0] = 1.0
+        self._tensors = [block[i] for i in range(self._nqudits)]
+
+    # ------------------------------------------------------------ properties
+    @property
+    def nqudits(self) -> int:
+        return self._nqudits
+
+    @property
+    def qudit_dimension(self) -> int:
+        return self._qudit_dimension
+
+    @property
+    def device(self):
+        return self._engine.dev
+
+    @property
+    def tensors(self):
+        """The device site tensors [chi_l, d, chi_r] (not copies)."""
+        return self._tensors
+
+    def bond_dimension_of(self, node_index: int) -> int:
+        """core.py:339-361: dimension of the bond right of node_index."""
+        if not self.is_valid():
+            raise ValueError("MPS is invalid.")
+        if node_index >= self._nqudits:
+            raise ValueError(f"Index should be less than {self._nqudits} but is {node_index}.")
+        if node_index == self._nqudits - 1 or node_index < -(self._nqudits - 1):
+            raise ValueError(f"Node {node_index} has no right bond.")
+        return int(self._tensors[node_index].shape[2])
+
+    def bond_dimensions(self) -> List[int]:
+        return [int(t.shape[2]) for t in self._tensors[:-1]]
+
+    def max_bond_dimension_of(self, edge_index: int) -> int:
+        if edge_index >= self._nqudits:
+            raise ValueError(f"Edge index should be less than {self._nqudits} but is {edge_index}.")
+        return self._max_bond_dimensions[edge_index]
+
+    def max_bond_dimensions(self) -> List[int]:
+        return self._max_bond_dimensions
+
+    def is_valid(self) -> bool:
+        """Structural check (core.py:388-419): n >= 2 tensors, physical legs
+        of size d, neighbouring bonds matching, unit end bonds."""
+        t = self._tensors
+        if len(t) < 2:
+            return False
+        if t[0].shape[0] != 1 or t[-1].shape[2] != 1:
+            return False
+        for i, x in enumerate(t):
+            if x.dim() != 3 or x.shape[1] != self._qudit_dimension:
+                return False
+            if i + 1 < len(t) and x.shape[2] != t[i + 1].shape[0]:
+                return False
+        return True
+
+    def get_nodes(self, copy: bool = True) -> List[Node]:
+        """Host copies of the site tensors as Nodes ([chi_l, d, chi_r])."""
+        return [Node(t.detach().cpu().numpy(), name=self._prefix + str(i))
+                for i, t in enumerate(self._tensors)]
+
+    def get_node(self, node_index: int, copy: bool = True) -> Node:
+        t = self._tensors[node_index]
+        i = node_index % self._nqudits
+        return Node(t.detach().cpu().numpy(), name=self._prefix + str(i))
+
+    # ------------------------------------------------------------ read-out
+    def wavefunction(self) -> np.ndarray:
+        """core.py:483-500: psi with site 0 the most significant digit."""
+        if not self.is_valid():
+            raise ValueError("MPS is not valid.")
+        return self._wavefunction_device().cpu().numpy()
+
+    def _wavefunction_device(self):
+        torch = self._engine.torch
+        d = self._qudit_dimension
+        t0 = self._tensors[0]
+        psi = t0.reshape(d, t0.shape[2])
+        for t in self._tensors[1:]:
+            cl, _, cr = t.shape
+            rows = psi.shape[0]
+            out = torch.empty((rows, d * cr), dtype=torch.complex64, device=self._engine.dev)
+            self._engine.gemm([dict(A=psi, B=t, C=out, m=rows, n=d * cr, k=cl, lda=psi.shape[1],
+                                    ldb=d * cr, ldc=d * cr)])
+            psi = out.reshape(rows * d, cr)
+        return psi.reshape(d ** self._nqudits)
+
+    def dagger(self) -> None:
+        """core.py:502-505: conjugate every tensor."""
+        self._engine.scale([t for t in self._tensors if t.numel()], 1.0, conj=True)
+
+    def inner_product(self, other: "MPS") -> complex:
+        """core.py:507-561. Like the reference (which conjugates `other`,
+        core.py:545-546) this returns sum_psi self(psi) * conj(other(psi))."""
+        if other._nqudits != self._nqudits:
+            raise ValueError("Cannot compute inner product: number of qudits must be equal.")
+        if other._qudit_dimension != self._qudit_dimension:
+            raise ValueError("Cannot compute inner product: qudit dimensions must be equal.")
+        if not self.is_valid():
+            raise ValueError("MPS is invalid.")
+        if not other.is_valid():
+            raise ValueError("Other MPS is invalid.")
+        return complex(self._inner_device(other).item())
+
+    def _inner_device(self, other: "MPS"):
+        """E[y, x] = sum conj(other) self over the sites left of the cut."""
+        torch = self._engine.torch
+        dev = self._engine.dev
+        d = self._qudit_dimension
+        env = torch.ones((1, 1), dtype=torch.complex64, device=dev)
+        for a, b in zip(self._tensors, other._tensors):
+            xl, _, xr = a.shape
+            yl, _, yr = b.shape
+            f = torch.empty((yl, d * xr), dtype=torch.complex64, device=dev)
+            e2 = torch.empty((yr, xr), dtype=torch.complex64, device=dev)
+            # F[y, (p, x')] = sum_x E[y, x] a[x, p, x']
+            self._engine.gemm([dict(A=env, B=a, C=f, m=yl, n=d * xr, k=xl, lda=xl, ldb=d * xr,
+                                    ldc=d * xr)])
+            # E'[y', x'] = sum_{y,p} conj(b[y, p, y']) F[y, p, x']
+            self._engine.gemm([dict(A=b, B=f, C=e2, m=yr, n=xr, k=yl * d, lda=yr, ldb=xr,
+                                    ldc=xr, flags=nat.GEMM_CONJ_A)])
+            env = e2
+        return env[0, 0]
+
+    def norm(self) -> float:
+        """core.py:563-565."""
+        return float(np.sqrt(self.inner_product(self).real))
+
+    def renormalize(self, to_norm: float = 1.0) -> None:
+        """core.py:567-594."""
+        if to_norm < 0.0:
+            raise ValueError(f"Arg to_norm must be positive but is {to_norm}")
+        if np.isclose(to_norm, 0.0, atol=1e-15):
+            raise ValueError(f"Arg to_norm = {to_norm} is too close to numerical zero.")
+        norm = self.norm()
+        if np.isclose(norm, 0.0, atol=1e-15):
+            raise ValueError("Norm of MPS is numerically zero, cannot renormalize.")
+        f = (to_norm / norm) ** (1.0 / self._nqudits)
+        self._engine.scale([t for t in self._tensors if t.numel()], f)
+
+    # ------------------------------------------------------------ one-site
+    def apply_one_qudit_gate(self, gate, node_index: int, **kwargs) -> None:
+        """core.py:753-845."""
+        if not self.is_valid():
+            raise ValueError("MPS is invalid.")
+        if node_index not in range(self._nqudits):
+            raise ValueError(f"Input tensor index={node_index} is out of bounds for"
+                             f" an MPS on {self._nqudits} qudits.")
+        nondangling = getattr(gate, "get_all_nondangling", None)
+        g = as_tensor(gate)
+        if g.ndim != 2 or (callable(nondangling) and len(nondangling()) != 0):
+            raise ValueError("Single qudit gate must have two free edges and zero connected edges.")
+        if g.shape[0] != g.shape[1]:
+            raise ValueError("Gate edge dimensions must be equal.")
+        if g.shape[0] != self._qudit_dimension:
+            raise ValueError(f"Gate edges have dimension {g.shape[0]} "
+                             f"but should have MPS qudit dimension = {self._qudit_dimension}")
+        unitary = is_unitary(g)
+        renorm = kwargs.get("renormalize_after_non_unitary") is not False
+        norm = self.norm() if (not unitary and renorm) else None
+        t = self._tensors[node_index]
+        if t.numel():
+            self._engine.one_site([(t, t, self._engine.gate(g))])
+        if norm is not None:
+            self.renormalize(norm)
+
+    def apply_one_qudit_gate_to_all(self, gate) -> None:
+        for i in range(self._nqudits):
+            self.apply_one_qudit_gate(gate, i)
+
+    # ------------------------------------------------------------ two-site
+    def apply_two_qudit_gate(self, gate, node_index1: int, node_index2: int, **kwargs) -> None:
+        """core.py:950-1161."""
+        if not self.is_valid():
+            raise ValueError("MPS is not valid.")
+        if node_index1 not in range(self._nqudits) or node_index2 not in range(self._nqudits):
+            raise ValueError(f"Input tensor indices={(node_index1, node_index2)} are out of "
+                             f"bounds for an MPS on {self._nqudits} qudits.")
+        if node_index1 == node_index2:
+            raise ValueError("Node indices cannot be identical.")
+        nondangling = getattr(gate, "get_all_nondangling", None)
+        g = as_tensor(gate)
+        d = self._qudit_dimension
+        if g.ndim != 4 or (callable(nondangling) and len(nondangling()) != 0):
+            raise ValueError("Two qubit gate must have four free edges and zero connected edges.")
+        if len(set(g.shape)) != 1:
+            raise ValueError("All gate edges must have the same dimension.")
+        if g.shape[0] != d:
+            raise ValueError(f"Gate edges have dimension {g.shape[0]} "
+                             f"but should have MPS qudit dimension = {d}")
+        # core.py:1031-1033: swap the roles of the two sites; like the
+        # reference, a Node passed in is reordered in place.
+        if node_index2 < node_index1:
+            if isinstance(gate, Node):
+                gate.reorder_edges([gate[1], gate[0], gate[3], gate[2]])
+                g = gate.tensor
+            else:
+                g = np.transpose(g, (1, 0, 3, 2))
+            node_index1, node_index2 = node_index2, node_index1
+        invert = False
+        if node_index1 < node_index2 - 1:
+            invert = True
+            original = node_index1
+            self.move_node_from_left_to_right(node_index1, node_index2 - 1, **kwargs)
+            node_index1 = node_index2 - 1
+        self._two_site(g.reshape(d * d, d * d), node_index1, kwargs)
+        if invert:
+            self.move_node_from_right_to_left(node_index1, original, **kwargs)
+        if self._track_norms:
+            self._norms.append(self.norm())
+
+    def _two_site(self, gmat: np.ndarray, i: int, kwargs) -> None:
+        maxsvals, keep_left = parse_split_options(kwargs, self.max_bond_dimension_of(i))
+        a, b = self._tensors[i], self._tensors[i + 1]
+        d = self._qudit_dimension
+        M, N = d * a.shape[0], d * b.shape[2]
+        k = min(M, N) if maxsvals is None else min(M, N, maxsvals)
+        item = TwoSiteItem(a, b, self._engine.gate(gmat), k, absorb_left=not keep_left)
+        res = self._engine.two_site([item])[0]
+        self._tensors[i] = res.a_out
+        self._tensors[i + 1] = res.b_out
+
+    def move_node_from_left_to_right(self, current_node_index: int, final_node_index: int,
+                                     **kwargs) -> None:
+        """core.py:1163-1190."""
+        if current_node_index > final_node_index:
+            raise ValueError("current_node_index should be smaller than final_node_index.")
+        if current_node_index < 0:
+            raise ValueError("current_node_index out of range.")
+        if final_node_index >= self._nqudits:
+            raise ValueError("final_node_index out of range.")
+        while current_node_index < final_node_index:
+            self.swap(current_node_index, current_node_index + 1, **kwargs)
+            current_node_index += 1
+
+    def move_node_from_right_to_left(self, current_node_index: int, final_node_index: int,
+                                     **kwargs) -> None:
+        """core.py:1192-1219."""
+        if current_node_index < final_node_index:
+            raise ValueError("current_node_index should be larger than final_node_index.")
+        if current_node_index > self._nqudits:
+            raise ValueError("current_node_index out of range.")
+        if final_node_index < 0:
+            raise ValueError("final_node_index out of range.")
+        while current_node_index > final_node_index:
+            self.swap(current_node_index - 1, current_node_index, **kwargs)
+            current_node_index -= 1
+
+    # ------------------------------------------------------------ dispatch
+    def apply(self, operations, **kwargs) -> None:
+        """core.py:1221-1247."""
+        try:
+            operations = iter(operations)
+        except TypeError:
+            operations = (operations,)
+        for op in operations:
+            self._apply_mps_operation(op, **kwargs)
+
+    def _apply_mps_operation(self, operation: MPSOperation, **kwargs) -> None:
+        """core.py:1249-1276."""
+        if not isinstance(operation, MPSOperation):
+            raise TypeError("Argument operation should be of type MPSOperation but is "
+                            f"of type {type(operation)}.")
+        if not operation.is_valid():
+            raise ValueError("Input MPS Operation is not valid.")
+        if operation.is_single_qudit_operation():
+            self.apply_one_qudit_gate(operation.node(), *operation.qudit_indices, **kwargs)
+        elif operation.is_two_qudit_operation():
+            self.apply_two_qudit_gate(operation.node(), *operation.qudit_indices, **kwargs)
+        else:
+            raise ValueError("Only one-qudit and two-qudit gates are supported. "
+                             "To apply a gate on three or more qudits, the gate must be "
+                             "compiled into a sequence of one- and two-qudit gates.")
+
+    # ------------------------------------------------------------ wrappers
+    def x(self, index: int) -> None:
+        if index == -1:
+            self.apply_one_qudit_gate_to_all(xgate())
+        else:
+            self.apply_one_qudit_gate(xgate(), index)
+
+    def h(self, index: int) -> None:
+        if index == -1:
+            self.apply_one_qudit_gate_to_all(hgate())
+        else:
+            self.apply_one_qudit_gate(hgate(), index)
+
+    def r(self, index, seed: Optional[int] = None, angle_scale: float = 1.0) -> None:
+        if index == -1:
+            for i in range(self._nqudits):
+                self.apply_one_qudit_gate(rgate(seed, angle_scale), i)
+        else:
+            self.apply_one_qudit_gate(rgate(seed, angle_scale), index)
+
+    def cnot(self, a: int, b: int, **kwargs) -> None:
+        self.apply_two_qudit_gate(cnot(), a, b, **kwargs)
+
+    def haar_random(self, qudit1_index: int, qudit2_index: int, **kwargs) -> None:
+        gate = haar_random_unitary(nqudits=2, qudit_dimension=self._qudit_dimension)
+        self.apply_two_qudit_gate(gate, qudit1_index, qudit2_index, **kwargs)
+
+    def sweep_haar_random_left_to_right(self, **kwargs) -> None:
+        for i in range(0, self._nqudits - 1, 2):
+            self.haar_random(i, i + 1, keep_left_canonical=True, **kwargs)
+
+    def sweep_haar_random_right_to_left(self, **kwargs) -> None:
+        for i in range(self._nqudits - 2, 0, -2):
+            self.haar_random(i - 1, i, keep_left_canonical=False, **kwargs)
+
+    def sweep_cnots_left_to_right(self, **kwargs) -> None:
+        for i in range(0, self._nqudits - 1, 2):
+            self.cnot(i, i + 1, keep_left_canonical=True, **kwargs)
+
+    def sweep_cnots_right_to_left(self, **kwargs) -> None:
+        for i in range(self._nqudits - 2, 0, -2):
+            self.cnot(i - 1, i, keep_left_canonical=False, **kwargs)
+
+    def swap(self, a: int, b: int, **kwargs) -> None:
+        if b < a:
+            a, b = b, a
+        self.apply_two_qudit_gate(swap(), a, b, **kwargs)
+
+    # ------------------------------------------------------------ misc
+    def copy(self) -> "MPS":
+        return self.__copy__()
+
+    def __copy__(self) -> "MPS":
+        new = MPS(self._nqudits, self._qudit_dimension, self._prefix, device=self._engine.dev,
+                  track_norms=self._track_norms, _empty=True)
+        new._tensors = [t.clone() for t in self._tensors]
+        new._norms = list(self._norms)
+        return new
+
+    def __eq__(self, other) -> bool:
+        """core.py:1389-1418."""
+        if not isinstance(other, MPS):
+            return False
+        if self is other:
+            return True
+        if not self.is_valid():
+            raise ValueError("MPS is invalid and cannot be compared to another MPS.")
+        if not other.is_valid():
+            raise ValueError("Other MPS is invalid.")
+        if other._qudit_dimension != self._qudit_dimension or other._nqudits != self._nqudits:
+            return False
+        for a, b in zip(self._tensors, other._tensors):
+            if tuple(a.shape) != tuple(b.shape):
+                return False
+            if not np.allclose(a.cpu().numpy(), b.cpu().numpy()):
+                return False
+        return True
+
+    __hash__ = None
+
+    def __str__(self) -> str:
+        return "----".join(f"{self._prefix}{i}{tuple(t.shape)}" for i, t in enumerate(self._tensors))

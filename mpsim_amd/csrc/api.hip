@@ -1,0 +1,319 @@
+// C ABI of libmpsx: argument checks, launch planning and the two-site
+// sweep driver. Kernels live in site.hip, gemm.hip and twosite.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "common.h"
+#include "launch.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(MPSX_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                      \
+    do {                                         \
+        hipError_t e_ = (expr);                  \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+constexpr int64_t kHeaderBytes = 1024;
+constexpr int kDefaultMaxSweeps = 30;
+constexpr int kDefaultInnerSweeps = 3;
+constexpr double kDefaultTol = 1e-5;
+constexpr double kNullRel = 1e-6;
+constexpr int64_t kGridLimit = 2147483647LL;
+
+struct Resolved {
+    int block, max_sweeps, check_every, inner_sweeps;
+    double tol;
+};
+
+Resolved resolve(const mpsx_svd_opts *o) {
+    Resolved r{8, kDefaultMaxSweeps, 1, kDefaultInnerSweeps, kDefaultTol};
+    if (o) {
+        if (o->block) r.block = o->block;
+        if (o->max_sweeps) r.max_sweeps = o->max_sweeps;
+        r.check_every = o->check_every;
+        if (o->inner_sweeps) r.inner_sweeps = o->inner_sweeps;
+        if (o->tol > 0) r.tol = o->tol;
+    }
+    return r;
+}
+
+int *poll_buffer() {
+    thread_local int *buf = nullptr;
+    if (!buf) {
+        void *p = nullptr;
+        if (hipHostMalloc(&p, 256, hipHostMallocDefault) != hipSuccess) return nullptr;
+        buf = static_cast<int *>(p);
+    }
+    return buf;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpsx_version(void) { return 100; }
+
+const char *mpsx_last_error(void) { return g_err.c_str(); }
+
+size_t mpsx_sizeof(int which) {
+    switch (which) {
+        case 0: return sizeof(mpsx_site_op);
+        case 1: return sizeof(mpsx_two_site_op);
+        case 2: return sizeof(mpsx_gemm_op);
+        case 3: return sizeof(mpsx_scale_op);
+        case 4: return sizeof(mpsx_svd_opts);
+        case 5: return sizeof(mpsx_plan);
+        default: return 0;
+    }
+}
+
+int mpsx_device_info(int device, char *arch, int arch_len, int *num_cus) {
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    if (arch && arch_len > 0) {
+        std::strncpy(arch, prop.gcnArchName, (size_t)arch_len - 1);
+        arch[arch_len - 1] = '\0';
+    }
+    if (num_cus) *num_cus = prop.multiProcessorCount;
+    return MPSX_OK;
+}
+
+// ---------------------------------------------------------------- one-site
+int mpsx_apply_one_site(mpsx_stream stream, const mpsx_site_op *h, const mpsx_site_op *d,
+                        int count) {
+    if (count < 0) return fail(MPSX_EINVAL, "count < 0");
+    if (count == 0) return MPSX_OK;
+    if (!h || !d) return fail(MPSX_EINVAL, "null descriptor table");
+    if (count > 65535) return fail(MPSX_EINVAL, "count %d > 65535", count);
+    int64_t max_pairs = 0;
+    int max_d = 0;
+    for (int i = 0; i < count; ++i) {
+        const mpsx_site_op &o = h[i];
+        if (o.d < 1 || o.d > 16) return fail(MPSX_EINVAL, "op %d: d=%d outside [1,16]", i, o.d);
+        if (o.chi_l < 0 || o.chi_r < 0) return fail(MPSX_EINVAL, "op %d: negative bond", i);
+        const int64_t pairs = o.chi_l * o.chi_r;
+        if (pairs > 0 && (!o.a || !o.out || !o.gate))
+            return fail(MPSX_EINVAL, "op %d: null pointer", i);
+        max_pairs = std::max(max_pairs, pairs);
+        max_d = std::max(max_d, o.d);
+    }
+    if (max_pairs == 0) return MPSX_OK;
+    HIP_TRY(launch_one_site((hipStream_t)stream, d, count, max_pairs, max_d), "k_one_site");
+    return MPSX_OK;
+}
+
+// ---------------------------------------------------------------- scale
+int mpsx_scale(mpsx_stream stream, const mpsx_scale_op *h, const mpsx_scale_op *d, int count) {
+    if (count < 0) return fail(MPSX_EINVAL, "count < 0");
+    if (count == 0) return MPSX_OK;
+    if (!h || !d) return fail(MPSX_EINVAL, "null descriptor table");
+    if (count > 65535) return fail(MPSX_EINVAL, "count %d > 65535", count);
+    int64_t max_n = 0;
+    for (int i = 0; i < count; ++i) {
+        if (h[i].n < 0) return fail(MPSX_EINVAL, "op %d: n < 0", i);
+        if (h[i].n > 0 && !h[i].x) return fail(MPSX_EINVAL, "op %d: null pointer", i);
+        max_n = std::max(max_n, h[i].n);
+    }
+    if (max_n == 0) return MPSX_OK;
+    HIP_TRY(launch_scale((hipStream_t)stream, d, count, max_n), "k_scale");
+    return MPSX_OK;
+}
+
+// ---------------------------------------------------------------- gemm
+int mpsx_gemm(mpsx_stream stream, const mpsx_gemm_op *h, const mpsx_gemm_op *d, int count) {
+    if (count < 0) return fail(MPSX_EINVAL, "count < 0");
+    if (count == 0) return MPSX_OK;
+    if (!h || !d) return fail(MPSX_EINVAL, "null descriptor table");
+    if (count > 65535) return fail(MPSX_EINVAL, "count %d > 65535", count);
+    int64_t max_m = 0, max_n = 0;
+    for (int i = 0; i < count; ++i) {
+        const mpsx_gemm_op &o = h[i];
+        if (o.m < 0 || o.n < 0 || o.k < 0) return fail(MPSX_EINVAL, "op %d: negative size", i);
+        const bool ta = o.flags & 1, tb = o.flags & 2;
+        if (o.m > 0 && o.n > 0) {
+            if (!o.C) return fail(MPSX_EINVAL, "op %d: null C", i);
+            if (o.k > 0 && (!o.A || !o.B)) return fail(MPSX_EINVAL, "op %d: null A/B", i);
+            if (o.ldc < o.n) return fail(MPSX_EINVAL, "op %d: ldc < n", i);
+            if (o.k > 0 && o.lda < (ta ? o.m : o.k)) return fail(MPSX_EINVAL, "op %d: lda too small", i);
+            if (o.k > 0 && o.ldb < (tb ? o.k : o.n)) return fail(MPSX_EINVAL, "op %d: ldb too small", i);
+        }
+        max_m = std::max(max_m, o.m);
+        max_n = std::max(max_n, o.n);
+    }
+    if (max_m == 0 || max_n == 0) return MPSX_OK;
+    if ((max_m + 31) / 32 > 65535) return fail(MPSX_EINVAL, "m too large for one launch");
+    HIP_TRY(launch_gemm((hipStream_t)stream, d, count, max_m, max_n), "k_gemm");
+    return MPSX_OK;
+}
+
+// ---------------------------------------------------------------- two-site
+int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts,
+                       size_t *bytes) {
+    if (count < 0) return fail(MPSX_EINVAL, "count < 0");
+    if (count > 0 && !h) return fail(MPSX_EINVAL, "null descriptor table");
+    const Resolved r = resolve(opts);
+    if (r.block != 4 && r.block != 8 && r.block != 16)
+        return fail(MPSX_EINVAL, "block %d not in {4, 8, 16}", r.block);
+    if (r.max_sweeps < 1 || r.max_sweeps > 64) return fail(MPSX_EINVAL, "max_sweeps outside [1,64]");
+    int64_t off = kHeaderBytes, tiles = 0, pairs = 0, rows = 0, outs = 0;
+    for (int i = 0; i < count; ++i) {
+        mpsx_two_site_op &o = h[i];
+        if (o.d < 2 || o.d > 4) return fail(MPSX_EINVAL, "op %d: d=%d not in {2,3,4}", i, o.d);
+        if (o.chi_l < 0 || o.chi_m < 0 || o.chi_r < 0)
+            return fail(MPSX_EINVAL, "op %d: negative bond dimension", i);
+        const int64_t M = (int64_t)o.d * o.chi_l, N = (int64_t)o.d * o.chi_r;
+        if (o.k < 0 || o.k > std::min(M, N))
+            return fail(MPSX_EINVAL, "op %d: k=%d outside [0, min(%lld,%lld)]", i, o.k,
+                        (long long)M, (long long)N);
+        if (M > 65536 || N > 65536) return fail(MPSX_EINVAL, "op %d: matrix too large", i);
+        if (M * N > 0 && o.chi_m > 0 && (!o.a || !o.b)) return fail(MPSX_EINVAL, "op %d: null input", i);
+        if (M * N > 0 && !o.gate) return fail(MPSX_EINVAL, "op %d: null gate", i);
+        if (M * o.k > 0 && !o.a_out) return fail(MPSX_EINVAL, "op %d: null a_out", i);
+        if (N * o.k > 0 && !o.b_out) return fail(MPSX_EINVAL, "op %d: null b_out", i);
+        const bool left_absorb = (o.flags & 1) != 0;
+        mpsx_plan &p = o.plan;
+        p.C = (int32_t)(left_absorb ? N : M);
+        p.R = (int32_t)(left_absorb ? M : N);
+        p.b = r.block;
+        if (p.C == 0) {
+            p.nb = 0;
+            p.Cp = 0;
+        } else {
+            int nb = (p.C + p.b - 1) / p.b;
+            if (nb < 2) nb = 2;
+            if (nb & 1) ++nb;
+            p.nb = nb;
+            p.Cp = nb * p.b;
+        }
+        p.LD = (int32_t)align_up((int64_t)p.R + p.Cp, 4);
+        p.ws_off = off;
+        p.tile_base = tiles;
+        p.pair_base = pairs;
+        p.row_base = rows;
+        p.out_base = outs;
+        off += problem_bytes(p);
+        tiles += ((int64_t)o.chi_l + THETA_TL - 1) / THETA_TL * (((int64_t)o.chi_r + THETA_TR - 1) / THETA_TR);
+        pairs += p.nb / 2;
+        rows += p.Cp;
+        outs += M + o.k;
+    }
+    if (tiles > kGridLimit || pairs > kGridLimit || rows > kGridLimit || outs > kGridLimit)
+        return fail(MPSX_EINVAL, "batch too large for one launch");
+    if (bytes) *bytes = (size_t)off;
+    g_err.clear();
+    return MPSX_OK;
+}
+
+int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two_site_op *d,
+                  int count, const mpsx_svd_opts *opts, void *ws_, size_t ws_bytes,
+                  int *sweeps_out) {
+    hipStream_t stream = (hipStream_t)stream_;
+    if (sweeps_out) *sweeps_out = 0;
+    if (count < 0) return fail(MPSX_EINVAL, "count < 0");
+    if (count == 0) return MPSX_OK;
+    if (!h || !d) return fail(MPSX_EINVAL, "null descriptor table");
+    if (!ws_) return fail(MPSX_EINVAL, "null workspace");
+    const Resolved r = resolve(opts);
+    char *ws = static_cast<char *>(ws_);
+
+    // Re-derive totals from the planned host table and check the plan is
+    // consistent (a table that was not planned, or edited, is rejected).
+    int64_t need = kHeaderBytes, tiles = 0, pairs = 0, rows = 0, outs = 0;
+    int max_nb = 0;
+    bool has_d[5] = {false, false, false, false, false};
+    bool has_b[17] = {false};
+    for (int i = 0; i < count; ++i) {
+        const mpsx_two_site_op &o = h[i];
+        const mpsx_plan &p = o.plan;
+        if (p.ws_off != need || p.tile_base != tiles || p.pair_base != pairs || p.row_base != rows ||
+            p.out_base != outs)
+            return fail(MPSX_EINVAL, "op %d: plan missing or stale (call mpsx_two_site_plan)", i);
+        if (o.d < 2 || o.d > 4 || (p.b != 4 && p.b != 8 && p.b != 16))
+            return fail(MPSX_EINVAL, "op %d: invalid plan", i);
+        need += problem_bytes(p);
+        tiles += ((int64_t)o.chi_l + THETA_TL - 1) / THETA_TL * (((int64_t)o.chi_r + THETA_TR - 1) / THETA_TR);
+        pairs += p.nb / 2;
+        rows += p.Cp;
+        outs += (int64_t)o.d * o.chi_l + o.k;
+        max_nb = std::max(max_nb, p.nb);
+        has_d[o.d] = true;
+        if (p.nb > 0) has_b[p.b] = true;
+    }
+    if ((int64_t)ws_bytes < need)
+        return fail(MPSX_EWORKSPACE, "workspace %zu bytes < %lld needed", ws_bytes, (long long)need);
+
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(stream, &cap), "hipStreamIsCapturing");
+    const bool can_poll = r.check_every > 0 && cap == hipStreamCaptureStatusNone;
+
+    if (rows > 0) HIP_TRY(launch_init_rows(stream, d, count, ws, rows), "k_init_rows");
+    for (int D = 2; D <= 4; ++D)
+        if (has_d[D] && tiles > 0) HIP_TRY(launch_theta(stream, D, d, count, ws, tiles), "k_theta");
+    if (rows > 0) {
+        HIP_TRY(launch_rownorm(stream, d, count, ws, rows), "k_rownorm");
+        HIP_TRY(launch_fro(stream, d, count, ws, kNullRel), "k_fro");
+    }
+
+    int sweeps = 0;
+    bool converged = false;
+    if (pairs > 0) {
+        int *pin = can_poll ? poll_buffer() : nullptr;
+        for (int s = 0; s < r.max_sweeps; ++s) {
+            for (int round = 0; round < max_nb - 1; ++round)
+                for (int b : {4, 8, 16})
+                    if (has_b[b])
+                        HIP_TRY(launch_jacobi_round(stream, b, d, count, ws, pairs, round, r.tol,
+                                                    r.tol * 0.01, r.inner_sweeps),
+                                "k_jacobi_round");
+            HIP_TRY(launch_sweep_end(stream, d, count, ws, s), "k_sweep_end");
+            ++sweeps;
+            if (pin && ((s + 1) % r.check_every == 0 || s + 1 == r.max_sweeps)) {
+                const TwoSiteHeader *hdr = reinterpret_cast<const TwoSiteHeader *>(ws);
+                HIP_TRY(hipMemcpyAsync(pin, &hdr->active[s & 63], sizeof(int), hipMemcpyDeviceToHost,
+                                       stream),
+                        "hipMemcpyAsync");
+                HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+                if (*pin == 0) {
+                    converged = true;
+                    break;
+                }
+            }
+        }
+    } else {
+        converged = true;
+    }
+
+    if (rows > 0) {
+        HIP_TRY(launch_rownorm(stream, d, count, ws, rows), "k_rownorm");
+        HIP_TRY(launch_sort(stream, d, count, ws), "k_sort");
+    }
+    if (outs > 0) HIP_TRY(launch_gather(stream, d, count, ws, outs), "k_gather");
+    if (sweeps_out) *sweeps_out = sweeps;
+    if (can_poll && !converged)
+        return fail(MPSX_ENOTCONV, "Jacobi SVD not converged after %d sweeps", sweeps);
+    return MPSX_OK;
+}
+
+}  // extern "C"

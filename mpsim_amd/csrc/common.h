@@ -1,0 +1,102 @@
+// Shared device helpers and the library-private plan layout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mpsx.h"
+
+#define MPSX_DEV __device__ __forceinline__
+#define MPSX_HD __host__ __device__ __forceinline__
+
+// ---------------------------------------------------------------- complex
+MPSX_DEV float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// c + a * b
+MPSX_DEV float2 cfma(float2 a, float2 b, float2 c) {
+    c.x = fmaf(a.x, b.x, c.x);
+    c.x = fmaf(-a.y, b.y, c.x);
+    c.y = fmaf(a.x, b.y, c.y);
+    c.y = fmaf(a.y, b.x, c.y);
+    return c;
+}
+// c + conj(a) * b
+MPSX_DEV float2 cfma_conj(float2 a, float2 b, float2 c) {
+    c.x = fmaf(a.x, b.x, c.x);
+    c.x = fmaf(a.y, b.y, c.x);
+    c.y = fmaf(a.x, b.y, c.y);
+    c.y = fmaf(-a.y, b.x, c.y);
+    return c;
+}
+MPSX_DEV float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+MPSX_DEV double2 zmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+MPSX_DEV double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+MPSX_DEV double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+MPSX_DEV double2 zscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+
+// Round-robin ("circle method") pairing of n (even) items: in round r
+// (0 <= r < n-1) slot p (0 <= p < n/2) pairs item lo with item hi; every
+// item appears once per round and every unordered pair meets exactly once
+// over the n-1 rounds.
+MPSX_HD int rr_item(int pos, int r, int n) {
+    return pos == 0 ? 0 : ((pos - 1 + r) % (n - 1)) + 1;
+}
+MPSX_HD void rr_pair(int p, int r, int n, int *lo, int *hi) {
+    int a = rr_item(p, r, n);
+    int b = rr_item(n - 1 - p, r, n);
+    *lo = a < b ? a : b;
+    *hi = a < b ? b : a;
+}
+
+// Binary search: index i with base[i] <= x < base[i+1] for a table of
+// `count` records of byte size `stride` whose int64 field at `field_off`
+// holds an exclusive prefix sum (records are sorted by construction).
+MPSX_DEV int find_record(const char *table, int count, size_t stride, size_t field_off,
+                         int64_t x) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        int64_t v = *reinterpret_cast<const int64_t *>(table + (size_t)mid * stride + field_off);
+        if (v <= x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Workspace header of a two-site batch (at ws + 0).
+struct TwoSiteHeader {
+    int32_t active[64];  // active[s]: problems still unconverged after sweep s
+    int32_t pad[64];
+};
+
+// Per-problem control block inside the workspace.
+struct ProblemCtl {
+    double fro2;    // ||theta||_F^2
+    double null2;   // rows with squared norm <= null2 are treated as zero
+    int32_t counter;  // pivot pairs rotated in the current sweep
+    int32_t conv;     // 1 once a full sweep rotated nothing
+    int32_t sweeps;   // sweeps that did work
+    int32_t pad;
+};
+
+// Byte layout of one problem's region starting at plan.ws_off:
+//   ProblemCtl | sig2[Cp] (double) | perm[Cp] (int32) | Z[Cp][LD] (float2)
+MPSX_HD int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+MPSX_HD int64_t ctl_bytes() { return 64; }
+MPSX_HD int64_t sig_off(const mpsx_plan &p) { return p.ws_off + ctl_bytes(); }
+MPSX_HD int64_t perm_off(const mpsx_plan &p) { return sig_off(p) + align_up(8LL * p.Cp, 256); }
+MPSX_HD int64_t z_off(const mpsx_plan &p) { return perm_off(p) + align_up(4LL * p.Cp, 256); }
+MPSX_HD int64_t problem_bytes(const mpsx_plan &p) {
+    return align_up(z_off(p) - p.ws_off + 8LL * p.Cp * p.LD, 256);
+}
+
+// Tile sizes of the theta kernel.
+constexpr int THETA_TL = 32;  // l values per tile
+constexpr int THETA_TR = 32;  // r values per tile
+constexpr int THETA_KT = 16;  // bond (m) values per LDS stage

@@ -1,0 +1,593 @@
+// Two-site update kernels for gfx950.
+//
+//   theta      fused  theta = G . (A . B)        (mpsim/core.py:1045-1068)
+//   jacobi     blocked one-sided Jacobi SVD of theta's rows
+//              (replaces np.linalg.svd inside TensorNetwork's
+//               split_node_full_svd, called at core.py:1132-1137)
+//   finalize   singular values, stable descending order, truncation to k,
+//              absorb of S (core.py:1139-1145), discarded weight.
+//
+// Working matrix. Let X be theta (absorb right) or theta^T (absorb left):
+// C rows of length R. The Jacobi iteration applies unitary 2b x 2b pivot
+// transforms Q^H to pairs of row blocks of Z = [X | Y], Y starting as the
+// identity, until the rows of X are mutually orthogonal. Then X = W Xt with
+// W = Y^H unitary and the rows of Xt orthogonal, so
+//   absorb right:  theta   = W Xt     -> U = W,  S V^H = Xt
+//   absorb left:   theta^T = W Xt     -> U S = Xt^T,  V^H = W^T
+// and the factor that carries S is read straight from Xt: nothing is ever
+// divided by a singular value, and the unscaled factor W is exactly unitary
+// also for the zero singular values the reference keeps.
+//
+// Precision: theta, X and Y are complex64. Pivot Gram matrices are
+// accumulated in FP32 within 64-column chunks and in FP64 across chunks; the
+// pivot eigensolve (cyclic Jacobi on the 2b x 2b Hermitian Gram), the
+// convergence test and the row norms / discarded weight are FP64.
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "common.h"
+#include "launch.h"
+
+#define OP_FIELD(f) offsetof(mpsx_two_site_op, f)
+
+MPSX_DEV const mpsx_two_site_op &op_at(const mpsx_two_site_op *ops, int i) { return ops[i]; }
+
+MPSX_DEV float2 *z_ptr(char *ws, const mpsx_plan &p) {
+    return reinterpret_cast<float2 *>(ws + z_off(p));
+}
+MPSX_DEV double *sig_ptr(char *ws, const mpsx_plan &p) {
+    return reinterpret_cast<double *>(ws + sig_off(p));
+}
+MPSX_DEV int *perm_ptr(char *ws, const mpsx_plan &p) {
+    return reinterpret_cast<int *>(ws + perm_off(p));
+}
+MPSX_DEV ProblemCtl *ctl_ptr(char *ws, const mpsx_plan &p) {
+    return reinterpret_cast<ProblemCtl *>(ws + p.ws_off);
+}
+
+// Block-wide FP64 sum (256 threads).
+MPSX_DEV double block_sum_256(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) s = red[0] + red[1] + red[2] + red[3];
+    return s;  // valid in thread 0
+}
+
+// ---------------------------------------------------------------- init
+// One workgroup per padded row of every problem: Y <- identity, padded X
+// rows <- 0. Row 0 of each problem also resets its control block.
+__global__ void __launch_bounds__(256)
+k_init_rows(const mpsx_two_site_op *ops, int count, char *ws) {
+    if (blockIdx.x == 0 && threadIdx.x < 128)
+        reinterpret_cast<TwoSiteHeader *>(ws)->active[threadIdx.x & 63] = 0;
+    const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
+                               OP_FIELD(plan.row_base), blockIdx.x);
+    const mpsx_plan p = ops[pi].plan;
+    const int row = (int)(blockIdx.x - p.row_base);
+    if (row >= p.Cp) return;
+    float2 *z = z_ptr(ws, p) + (int64_t)row * p.LD;
+    if (row >= p.C)
+        for (int c = threadIdx.x; c < p.R; c += 256) z[c] = make_float2(0.f, 0.f);
+    for (int c = threadIdx.x; c < p.Cp; c += 256)
+        z[p.R + c] = make_float2(c == row ? 1.f : 0.f, 0.f);
+    if (row == 0 && threadIdx.x == 0) {
+        ProblemCtl *ctl = ctl_ptr(ws, p);
+        ctl->fro2 = 0.0;
+        ctl->null2 = 0.0;
+        ctl->counter = 0;
+        ctl->conv = (p.C == 0 || p.R == 0) ? 1 : 0;
+        ctl->sweeps = 0;
+    }
+}
+
+// ---------------------------------------------------------------- theta
+// One workgroup per 32x32 (l, r) tile; thread (ty, tx) owns l in
+// {ty, ty+16}, r in {tx, tx+16} and all D*D (p1, p2) partial products.
+template <int D>
+__global__ void __launch_bounds__(256)
+k_theta(const mpsx_two_site_op *ops, int count, char *ws) {
+    constexpr int TL = THETA_TL, TR = THETA_TR, KT = THETA_KT;
+    __shared__ float2 As[TL * D][KT + 1];
+    __shared__ float2 Bs[KT * D][TR + 1];
+    __shared__ float2 Gs[D * D * D * D];
+
+    const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
+                               OP_FIELD(plan.tile_base), blockIdx.x);
+    const mpsx_two_site_op &op = ops[pi];
+    if (op.d != D) return;
+    const int chi_l = op.chi_l, chi_m = op.chi_m, chi_r = op.chi_r;
+    const mpsx_plan p = op.plan;
+    const int tiles_r = (chi_r + TR - 1) / TR;
+    const int t = (int)(blockIdx.x - p.tile_base);
+    const int l0 = (t / tiles_r) * TL, r0 = (t % tiles_r) * TR;
+    const float2 *A = reinterpret_cast<const float2 *>(op.a);
+    const float2 *B = reinterpret_cast<const float2 *>(op.b);
+    const float2 *G = reinterpret_cast<const float2 *>(op.gate);
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+
+    for (int i = tid; i < D * D * D * D; i += 256) Gs[i] = G[i];
+
+    float2 acc[2][2][D][D];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j) acc[a][b][i][j] = make_float2(0.f, 0.f);
+
+    for (int m0 = 0; m0 < chi_m; m0 += KT) {
+        for (int e = tid; e < TL * D * KT; e += 256) {
+            const int km = e % KT, row = e / KT;  // row = il*D + p1
+            const int l = l0 + row / D, m = m0 + km;
+            float2 v = make_float2(0.f, 0.f);
+            if (l < chi_l && m < chi_m) v = A[((int64_t)l0 * D + row) * chi_m + m];
+            As[row][km] = v;
+        }
+        for (int e = tid; e < KT * D * TR; e += 256) {
+            const int ir = e % TR, row = e / TR;  // row = km*D + p2
+            const int m = m0 + row / D, r = r0 + ir;
+            float2 v = make_float2(0.f, 0.f);
+            if (m < chi_m && r < chi_r) v = B[((int64_t)m0 * D + row) * chi_r + r];
+            Bs[row][ir] = v;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int km = 0; km < KT; ++km) {
+            float2 av[2][D], bv[2][D];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int i = 0; i < D; ++i) av[a][i] = As[(ty + 16 * a) * D + i][km];
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int j = 0; j < D; ++j) bv[b][j] = Bs[km * D + j][tx + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int i = 0; i < D; ++i)
+#pragma unroll
+                        for (int j = 0; j < D; ++j)
+                            acc[a][b][i][j] = cfma(av[a][i], bv[b][j], acc[a][b][i][j]);
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+
+    // Epilogue: theta[l,o1,o2,r] = sum G[(o1,o2),(p1,p2)] T[l,p1,p2,r]
+    float2 *Z = z_ptr(ws, p);
+    const int LD = p.LD;
+    const bool transposed = (op.flags & 1) != 0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int l = l0 + ty + 16 * a;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int r = r0 + tx + 16 * b;
+            if (l >= chi_l || r >= chi_r) continue;
+#pragma unroll
+            for (int o1 = 0; o1 < D; ++o1)
+#pragma unroll
+                for (int o2 = 0; o2 < D; ++o2) {
+                    float2 v = make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int i = 0; i < D; ++i)
+#pragma unroll
+                        for (int j = 0; j < D; ++j)
+                            v = cfma(Gs[(o1 * D + o2) * D * D + i * D + j], acc[a][b][i][j], v);
+                    const int64_t row = (int64_t)l * D + o1;  // theta row (l, o1)
+                    const int64_t col = (int64_t)o2 * chi_r + r;  // theta col (o2, r)
+                    if (!transposed)
+                        Z[row * LD + col] = v;
+                    else
+                        Z[col * LD + row] = v;
+                }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- norms
+// One workgroup per padded row: sig2[row] = sum_c |X[row, c]|^2 (FP64).
+__global__ void __launch_bounds__(256)
+k_rownorm(const mpsx_two_site_op *ops, int count, char *ws) {
+    __shared__ double red[4];
+    const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
+                               OP_FIELD(plan.row_base), blockIdx.x);
+    const mpsx_plan p = ops[pi].plan;
+    const int row = (int)(blockIdx.x - p.row_base);
+    if (row >= p.Cp) return;
+    const float2 *z = z_ptr(ws, p) + (int64_t)row * p.LD;
+    double s = 0.0;
+    if (row < p.C)
+        for (int c = threadIdx.x; c < p.R; c += 256) {
+            const float2 v = z[c];
+            s += (double)v.x * v.x + (double)v.y * v.y;
+        }
+    s = block_sum_256(s, red);
+    if (threadIdx.x == 0) sig_ptr(ws, p)[row] = s;
+}
+
+// One workgroup per problem: Frobenius norm and the null-row threshold.
+__global__ void __launch_bounds__(256)
+k_fro(const mpsx_two_site_op *ops, int count, char *ws, double null_rel) {
+    __shared__ double red[4];
+    const int pi = blockIdx.x;
+    if (pi >= count) return;
+    const mpsx_plan p = ops[pi].plan;
+    const double *sig = sig_ptr(ws, p);
+    double s = 0.0;
+    for (int i = threadIdx.x; i < p.C; i += 256) s += sig[i];
+    s = block_sum_256(s, red);
+    if (threadIdx.x == 0) {
+        ProblemCtl *ctl = ctl_ptr(ws, p);
+        ctl->fro2 = s;
+        ctl->null2 = null_rel * null_rel * s;
+    }
+}
+
+// ---------------------------------------------------------------- Jacobi
+// One workgroup per (problem, pivot pair) of round `round`. B2 = 2b rows.
+template <int B2>
+__global__ void __launch_bounds__(256)
+k_jacobi_round(const mpsx_two_site_op *ops, int count, char *ws, int round, double tol,
+               double tol_inner, int inner_sweeps) {
+    constexpr int b = B2 / 2;
+    constexpr int KC = 64;                       // columns per Gram stage
+    constexpr int NBLK = (B2 / 2) * (B2 / 2);    // 2x2 Gram blocks
+    constexpr int SLICES = 256 / NBLK;           // K-split of the Gram
+    static_assert(SLICES >= 1, "pivot too large");
+
+    __shared__ float2 Xs[B2][KC + 1];
+    __shared__ double2 Hp[SLICES][B2][B2];
+    __shared__ double2 H[B2][B2];
+    __shared__ double2 Q[B2][B2];
+    __shared__ float2 Qf[B2][B2];
+    __shared__ int rows[B2];
+    __shared__ double prm_c[b];
+    __shared__ double2 prm_s[b];
+    __shared__ int prm_p[b], prm_q[b];
+    __shared__ int s_any;
+
+    const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
+                               OP_FIELD(plan.pair_base), blockIdx.x);
+    const mpsx_plan p = ops[pi].plan;
+    if (p.b != b) return;
+    if (round >= p.nb - 1) return;
+    ProblemCtl *ctl = ctl_ptr(ws, p);
+    if (ctl->conv) return;
+    const double null2 = ctl->null2;
+    const int pair = (int)(blockIdx.x - p.pair_base);
+    int I, J;
+    rr_pair(pair, round, p.nb, &I, &J);
+    const int tid = threadIdx.x;
+    if (tid < B2) rows[tid] = tid < b ? I * b + tid : J * b + (tid - b);
+    float2 *Z = z_ptr(ws, p);
+    const int LD = p.LD, R = p.R;
+    __syncthreads();
+
+    // ---- Gram of the pivot rows: H = Xp Xp^H
+    const int blk = tid % NBLK, slice = tid / NBLK;
+    const int bi = blk / (B2 / 2), bj = blk % (B2 / 2);
+    double2 acc64[2][2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) acc64[u][v] = make_double2(0.0, 0.0);
+    for (int c0 = 0; c0 < R; c0 += KC) {
+        for (int e = tid; e < B2 * KC; e += 256) {
+            const int rr = e / KC, cc = e % KC;
+            const int c = c0 + cc;
+            Xs[rr][cc] = c < R ? Z[(int64_t)rows[rr] * LD + c] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        if (slice < SLICES) {
+            float2 acc32[2][2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int v = 0; v < 2; ++v) acc32[u][v] = make_float2(0.f, 0.f);
+            for (int cc = slice; cc < KC; cc += SLICES) {
+                const float2 x0 = Xs[2 * bi][cc], x1 = Xs[2 * bi + 1][cc];
+                const float2 y0 = Xs[2 * bj][cc], y1 = Xs[2 * bj + 1][cc];
+                // H_ij += x_i * conj(x_j)
+                acc32[0][0] = cfma_conj(y0, x0, acc32[0][0]);
+                acc32[0][1] = cfma_conj(y1, x0, acc32[0][1]);
+                acc32[1][0] = cfma_conj(y0, x1, acc32[1][0]);
+                acc32[1][1] = cfma_conj(y1, x1, acc32[1][1]);
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int v = 0; v < 2; ++v) {
+                    acc64[u][v].x += acc32[u][v].x;
+                    acc64[u][v].y += acc32[u][v].y;
+                }
+        }
+        __syncthreads();
+    }
+    if (slice < SLICES) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 2; ++v) Hp[slice][2 * bi + u][2 * bj + v] = acc64[u][v];
+    }
+    __syncthreads();
+    for (int e = tid; e < B2 * B2; e += 256) {
+        const int i = e / B2, j = e % B2;
+        double2 s = make_double2(0.0, 0.0);
+        if (i <= j) {
+#pragma unroll
+            for (int sl = 0; sl < SLICES; ++sl) s = zadd(s, Hp[sl][i][j]);
+        } else {
+#pragma unroll
+            for (int sl = 0; sl < SLICES; ++sl) s = zadd(s, zconj(Hp[sl][j][i]));
+        }
+        if (i == j) s.y = 0.0;
+        H[i][j] = s;
+        Q[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+
+    // ---- convergence test on every off-diagonal pair of the pivot (the
+    // blocks' own rows included: a block is only orthogonal internally once
+    // it has been part of a rotated pivot)
+    int rot = 0;
+    for (int e = tid; e < B2 * B2; e += 256) {
+        const int i = e / B2, j = e % B2;
+        if (i >= j) continue;
+        const double hi = H[i][i].x, hj = H[j][j].x;
+        const double2 h = H[i][j];
+        const double g2 = h.x * h.x + h.y * h.y;
+        if (hi > null2 && hj > null2 && g2 > tol * tol * hi * hj) rot = 1;
+    }
+    if (!__syncthreads_or(rot)) return;
+    if (tid == 0) atomicAdd(&ctl->counter, 1);
+
+    // ---- pivot eigensolve: cyclic Jacobi on H (FP64), Q accumulates
+    for (int sw = 0; sw < inner_sweeps; ++sw) {
+        __syncthreads();  // every wave has read s_any of the previous sweep
+        if (tid == 0) s_any = 0;
+        __syncthreads();
+        for (int step = 0; step < B2 - 1; ++step) {
+            if (tid < b) {
+                int pp, qq;
+                rr_pair(tid, step, B2, &pp, &qq);
+                const double a = H[pp][pp].x, c = H[qq][qq].x;
+                const double2 h = H[pp][qq];
+                const double g = sqrt(h.x * h.x + h.y * h.y);
+                double cs = 1.0;
+                double2 s = make_double2(0.0, 0.0);
+                if (a > null2 && c > null2 && g > tol_inner * sqrt(a * c) && g > 0.0) {
+                    const double zeta = (c - a) / (2.0 * g);
+                    const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    cs = 1.0 / sqrt(1.0 + t * t);
+                    const double sn = t * cs;
+                    s = make_double2(sn * h.x / g, sn * h.y / g);  // sn * e^{i phi}
+                    s_any = 1;
+                }
+                prm_p[tid] = pp;
+                prm_q[tid] = qq;
+                prm_c[tid] = cs;
+                prm_s[tid] = s;
+            }
+            __syncthreads();
+            // H <- J^H H J on 2x2 blocks, Q <- Q J on column pairs.
+            // J = [[c, s], [-conj(s), c]] for each index pair (p, q).
+            constexpr int NH = b * b, NQ = B2 * b;
+            for (int it = tid; it < NH + NQ; it += 256) {
+                if (it < NH) {
+                    const int k1 = it / b, k2 = it % b;
+                    const int p1 = prm_p[k1], q1 = prm_q[k1], p2 = prm_p[k2], q2 = prm_q[k2];
+                    const double c1 = prm_c[k1], c2 = prm_c[k2];
+                    const double2 s1 = prm_s[k1], s2 = prm_s[k2];
+                    const double2 h00 = H[p1][p2], h01 = H[p1][q2], h10 = H[q1][p2], h11 = H[q1][q2];
+                    const double2 ms2 = make_double2(-s2.x, s2.y);  // -conj(s2)
+                    const double2 m00 = zadd(zscale(h00, c2), zmul(h01, ms2));
+                    const double2 m01 = zadd(zmul(h00, s2), zscale(h01, c2));
+                    const double2 m10 = zadd(zscale(h10, c2), zmul(h11, ms2));
+                    const double2 m11 = zadd(zmul(h10, s2), zscale(h11, c2));
+                    const double2 ns1 = make_double2(-s1.x, -s1.y);  // -s1
+                    const double2 cs1 = zconj(s1);
+                    H[p1][p2] = zadd(zscale(m00, c1), zmul(ns1, m10));
+                    H[p1][q2] = zadd(zscale(m01, c1), zmul(ns1, m11));
+                    H[q1][p2] = zadd(zmul(cs1, m00), zscale(m10, c1));
+                    H[q1][q2] = zadd(zmul(cs1, m01), zscale(m11, c1));
+                } else {
+                    const int q = it - NH;
+                    const int row = q / b, k = q % b;
+                    const int pp = prm_p[k], qq = prm_q[k];
+                    const double c = prm_c[k];
+                    const double2 s = prm_s[k];
+                    const double2 qp = Q[row][pp], qv = Q[row][qq];
+                    const double2 ms = make_double2(-s.x, s.y);  // -conj(s)
+                    Q[row][pp] = zadd(zscale(qp, c), zmul(qv, ms));
+                    Q[row][qq] = zadd(zmul(qp, s), zscale(qv, c));
+                }
+            }
+            __syncthreads();
+        }
+        if (!s_any) break;
+    }
+    for (int e = tid; e < B2 * B2; e += 256) {
+        const int i = e / B2, j = e % B2;
+        Qf[i][j] = make_float2((float)Q[i][j].x, (float)Q[i][j].y);
+    }
+    __syncthreads();
+
+    // ---- apply: Zp <- Q^H Zp over the whole row (X and Y parts)
+    const int W = p.R + p.Cp;
+    for (int c = tid; c < W; c += 256) {
+        float2 zout[B2];
+#pragma unroll
+        for (int j = 0; j < B2; ++j) zout[j] = make_float2(0.f, 0.f);
+        // Q stays in LDS (broadcast reads); each thread owns column c.
+#pragma unroll 2
+        for (int i = 0; i < B2; ++i) {
+            const float2 zi = Z[(int64_t)rows[i] * LD + c];
+#pragma unroll
+            for (int j = 0; j < B2; ++j) zout[j] = cfma_conj(Qf[i][j], zi, zout[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < B2; ++j) Z[(int64_t)rows[j] * LD + c] = zout[j];
+    }
+}
+
+// After each sweep: a problem whose sweep rotated nothing is converged.
+__global__ void k_sweep_end(const mpsx_two_site_op *ops, int count, char *ws, int sweep) {
+    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= count) return;
+    const mpsx_plan p = ops[pi].plan;
+    ProblemCtl *ctl = ctl_ptr(ws, p);
+    if (ctl->conv) return;
+    if (ctl->counter == 0) {
+        ctl->conv = 1;
+    } else {
+        ctl->sweeps += 1;
+        ctl->counter = 0;
+        atomicAdd(&reinterpret_cast<TwoSiteHeader *>(ws)->active[sweep & 63], 1);
+    }
+}
+
+// ---------------------------------------------------------------- finalize
+// One workgroup per problem: stable descending order of the row norms
+// (ties keep the lower row index), singular values, discarded weight.
+constexpr int SORT_LDS = 4096;
+__global__ void __launch_bounds__(256)
+k_sort(const mpsx_two_site_op *ops, int count, char *ws) {
+    __shared__ double sg[SORT_LDS];
+    __shared__ double red[4];
+    const int pi = blockIdx.x;
+    if (pi >= count) return;
+    const mpsx_two_site_op &op = ops[pi];
+    const mpsx_plan p = op.plan;
+    const int C = p.C, k = op.k;
+    const double *sig = sig_ptr(ws, p);
+    int *perm = perm_ptr(ws, p);
+    const bool in_lds = C <= SORT_LDS;
+    if (in_lds)
+        for (int i = threadIdx.x; i < C; i += 256) sg[i] = sig[i];
+    __syncthreads();
+    double disc = 0.0;
+    for (int i = threadIdx.x; i < C; i += 256) {
+        const double si = in_lds ? sg[i] : sig[i];
+        int rank = 0;
+        for (int j = 0; j < C; ++j) {
+            const double sj = in_lds ? sg[j] : sig[j];
+            rank += (sj > si) || (sj == si && j < i);
+        }
+        perm[rank] = i;
+        if (rank >= k) disc += si;
+    }
+    disc = block_sum_256(disc, red);
+    __syncthreads();
+    if (threadIdx.x == 0 && op.discarded) *op.discarded = disc;
+    if (op.s_out)
+        for (int j = threadIdx.x; j < k; j += 256) {
+            const double s2 = in_lds ? sg[perm[j]] : sig[perm[j]];
+            op.s_out[j] = (float)sqrt(s2);
+        }
+}
+
+// One workgroup per output row (d*chi_l rows of a_out, then k rows of b_out).
+__global__ void __launch_bounds__(256)
+k_gather(const mpsx_two_site_op *ops, int count, char *ws) {
+    const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
+                               OP_FIELD(plan.out_base), blockIdx.x);
+    const mpsx_two_site_op &op = ops[pi];
+    const mpsx_plan p = op.plan;
+    const int M = op.d * op.chi_l, N = op.d * op.chi_r, k = op.k;
+    const int o = (int)(blockIdx.x - p.out_base);
+    if (o >= M + k) return;
+    const float2 *Z = z_ptr(ws, p);
+    const int *perm = perm_ptr(ws, p);
+    const int LD = p.LD, R = p.R;
+    const bool left_absorb = (op.flags & 1) != 0;
+    if (o < M) {
+        const int m = o;
+        float2 *out = reinterpret_cast<float2 *>(op.a_out) + (int64_t)m * k;
+        for (int j = threadIdx.x; j < k; j += 256) {
+            const int64_t src = (int64_t)perm[j] * LD;
+            out[j] = left_absorb ? Z[src + m] : cconj(Z[src + R + m]);
+        }
+    } else {
+        const int j = o - M;
+        float2 *out = reinterpret_cast<float2 *>(op.b_out) + (int64_t)j * N;
+        const int64_t src = (int64_t)perm[j] * LD;
+        for (int n = threadIdx.x; n < N; n += 256)
+            out[n] = left_absorb ? cconj(Z[src + R + n]) : Z[src + n];
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_init_rows(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
+                            int64_t rows) {
+    hipLaunchKernelGGL(k_init_rows, dim3((unsigned)rows), dim3(256), 0, s, d, count, ws);
+    return hipGetLastError();
+}
+
+hipError_t launch_theta(hipStream_t s, int D, const mpsx_two_site_op *d, int count, char *ws,
+                        int64_t tiles) {
+    switch (D) {
+        case 2: hipLaunchKernelGGL(k_theta<2>, dim3((unsigned)tiles), dim3(256), 0, s, d, count, ws); break;
+        case 3: hipLaunchKernelGGL(k_theta<3>, dim3((unsigned)tiles), dim3(256), 0, s, d, count, ws); break;
+        case 4: hipLaunchKernelGGL(k_theta<4>, dim3((unsigned)tiles), dim3(256), 0, s, d, count, ws); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rownorm(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int64_t rows) {
+    hipLaunchKernelGGL(k_rownorm, dim3((unsigned)rows), dim3(256), 0, s, d, count, ws);
+    return hipGetLastError();
+}
+
+hipError_t launch_fro(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, double null_rel) {
+    hipLaunchKernelGGL(k_fro, dim3((unsigned)count), dim3(256), 0, s, d, count, ws, null_rel);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_round(hipStream_t s, int b, const mpsx_two_site_op *d, int count, char *ws,
+                               int64_t pairs, int round, double tol, double tol_inner,
+                               int inner_sweeps) {
+    switch (b) {
+        case 4:
+            hipLaunchKernelGGL(k_jacobi_round<8>, dim3((unsigned)pairs), dim3(256), 0, s, d, count, ws,
+                               round, tol, tol_inner, inner_sweeps);
+            break;
+        case 8:
+            hipLaunchKernelGGL(k_jacobi_round<16>, dim3((unsigned)pairs), dim3(256), 0, s, d, count, ws,
+                               round, tol, tol_inner, inner_sweeps);
+            break;
+        case 16:
+            hipLaunchKernelGGL(k_jacobi_round<32>, dim3((unsigned)pairs), dim3(256), 0, s, d, count, ws,
+                               round, tol, tol_inner, inner_sweeps);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_end(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int sweep) {
+    hipLaunchKernelGGL(k_sweep_end, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, d, count, ws,
+                       sweep);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws) {
+    hipLaunchKernelGGL(k_sort, dim3((unsigned)count), dim3(256), 0, s, d, count, ws);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int64_t outs) {
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)outs), dim3(256), 0, s, d, count, ws);
+    return hipGetLastError();
+}

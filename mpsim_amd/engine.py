@@ -1,0 +1,191 @@
+"""Per-device executor: builds descriptor tables and calls libmpsx.
+
+Every arithmetic step of the MPS path goes through here to the HIP library;
+there is no other implementation to fall back on.
+"""
+from __future__ import annotations
+
+import threading
+import warnings
+from collections import OrderedDict
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _native as nat
+
+
+@dataclass
+class SvdConfig:
+    """Jacobi SVD knobs (see mpsx_svd_opts in include/mpsx.h)."""
+    block: int = 0          # 0 = library default
+    max_sweeps: int = 0
+    check_every: int = 1
+    inner_sweeps: int = 0
+    tol: float = 0.0
+
+    def to_c(self) -> nat.SvdOpts:
+        return nat.SvdOpts(self.block, self.max_sweeps, self.check_every, self.inner_sweeps,
+                           self.tol)
+
+
+@dataclass
+class TwoSiteItem:
+    a: "object"            # torch [chi_l, d, chi_m]
+    b: "object"            # torch [chi_m, d, chi_r]
+    gate: "object"         # torch complex64 [d^2, d^2] on device
+    k: int
+    absorb_left: bool = False
+    want_s: bool = False
+
+
+@dataclass
+class TwoSiteResult:
+    a_out: "object"
+    b_out: "object"
+    s: Optional["object"]
+    discarded: "object"    # torch float64 scalar on device
+
+
+class Engine:
+    """Owns the device, a gate cache and a grow-only workspace."""
+
+    _engines: Dict[int, "Engine"] = {}
+    _lock = threading.Lock()
+
+    def __init__(self, dev):
+        import torch
+        self.torch = torch
+        self.lib = nat.load()
+        self.dev = nat.device(dev)
+        self._ws = None
+        self._gates: "OrderedDict[bytes, object]" = OrderedDict()
+        self.svd = SvdConfig()
+        self.last_sweeps = 0
+
+    @classmethod
+    def get(cls, dev=None) -> "Engine":
+        d = nat.device(dev)
+        idx = d.index if d.index is not None else 0
+        with cls._lock:
+            eng = cls._engines.get(idx)
+            if eng is None:
+                eng = cls(d)
+                cls._engines[idx] = eng
+            return eng
+
+    # ------------------------------------------------------------ helpers
+    @property
+    def stream(self) -> int:
+        return nat.stream_handle(self.dev)
+
+    def workspace(self, nbytes: int):
+        if self._ws is None or self._ws.numel() < nbytes:
+            grow = max(nbytes, 0 if self._ws is None else int(self._ws.numel() * 1.25))
+            self._ws = self.torch.empty(max(grow, 1024), dtype=self.torch.uint8, device=self.dev)
+        return self._ws
+
+    def gate(self, mat: np.ndarray):
+        """Device complex64 copy of a gate matrix (cached by content)."""
+        m = np.ascontiguousarray(np.asarray(mat, dtype=np.complex64))
+        key = m.tobytes() + bytes(str(m.shape), "ascii")
+        g = self._gates.get(key)
+        if g is None:
+            g = self.torch.from_numpy(m.copy()).to(self.dev)
+            self._gates[key] = g
+            if len(self._gates) > 8192:
+                self._gates.popitem(last=False)
+        else:
+            self._gates.move_to_end(key)
+        return g
+
+    # ------------------------------------------------------------ one-site
+    def one_site(self, items: Sequence[Tuple[object, object, object]]) -> None:
+        """items: (tensor_in, tensor_out, gate_dev [d, d]); out may be in."""
+        if not items:
+            return
+        tab = np.zeros(len(items), dtype=nat.SITE_OP)
+        for i, (a, out, g) in enumerate(items):
+            cl, d, cr = a.shape
+            tab[i] = (nat.ptr(a), nat.ptr(out), nat.ptr(g), cl, cr, d, 0)
+        dtab = nat.to_device_table(tab, self.dev)
+        nat.check(self.lib.mpsx_apply_one_site(self.stream, tab.ctypes.data, dtab.data_ptr(),
+                                               len(items)))
+        del dtab
+
+    # ------------------------------------------------------------ two-site
+    def two_site(self, items: Sequence[TwoSiteItem], svd: Optional[SvdConfig] = None
+                 ) -> List[TwoSiteResult]:
+        torch = self.torch
+        n = len(items)
+        if n == 0:
+            return []
+        tab = np.zeros(n, dtype=nat.TWO_SITE_OP)
+        outs: List[TwoSiteResult] = []
+        disc = torch.empty(n, dtype=torch.float64, device=self.dev)
+        for i, it in enumerate(items):
+            cl, d, cm = it.a.shape
+            cm2, d2, cr = it.b.shape
+            if cm != cm2 or d != d2:
+                raise ValueError(f"site tensors do not share a bond: {tuple(it.a.shape)} {tuple(it.b.shape)}")
+            a_out = torch.empty((cl, d, it.k), dtype=torch.complex64, device=self.dev)
+            b_out = torch.empty((it.k, d, cr), dtype=torch.complex64, device=self.dev)
+            s = torch.empty(it.k, dtype=torch.float32, device=self.dev) if it.want_s else None
+            tab[i]["a"] = nat.ptr(it.a)
+            tab[i]["b"] = nat.ptr(it.b)
+            tab[i]["gate"] = nat.ptr(it.gate)
+            tab[i]["a_out"] = nat.ptr(a_out)
+            tab[i]["b_out"] = nat.ptr(b_out)
+            tab[i]["s_out"] = nat.ptr(s)
+            tab[i]["discarded"] = int(disc.data_ptr()) + 8 * i
+            tab[i]["chi_l"], tab[i]["chi_m"], tab[i]["chi_r"] = cl, cm, cr
+            tab[i]["d"], tab[i]["k"] = d, it.k
+            tab[i]["flags"] = 1 if it.absorb_left else 0
+            outs.append(TwoSiteResult(a_out, b_out, s, disc[i]))
+        cfg = (svd or self.svd).to_c()
+        import ctypes
+        nbytes = ctypes.c_size_t(0)
+        nat.check(self.lib.mpsx_two_site_plan(tab.ctypes.data, n, ctypes.byref(cfg),
+                                              ctypes.byref(nbytes)))
+        ws = self.workspace(nbytes.value)
+        dtab = nat.to_device_table(tab, self.dev)
+        sweeps = ctypes.c_int(0)
+        code = self.lib.mpsx_two_site(self.stream, tab.ctypes.data, dtab.data_ptr(), n,
+                                      ctypes.byref(cfg), ws.data_ptr(), ws.numel(),
+                                      ctypes.byref(sweeps))
+        self.last_sweeps = sweeps.value
+        if code == nat.MPSX_ENOTCONV:
+            warnings.warn(nat.load().mpsx_last_error().decode(), nat.NotConvergedWarning)
+        else:
+            nat.check(code)
+        # keep the device table alive until the stream has consumed it
+        self._last_table = dtab
+        return outs
+
+    # ------------------------------------------------------------ gemm
+    def gemm(self, ops: Sequence[dict]) -> None:
+        """ops: dicts with A, B, C (torch), m, n, k, lda, ldb, ldc, flags,
+        alpha (complex)."""
+        if not ops:
+            return
+        tab = np.zeros(len(ops), dtype=nat.GEMM_OP)
+        for i, o in enumerate(ops):
+            al = complex(o.get("alpha", 1.0))
+            tab[i] = (nat.ptr(o["A"]), nat.ptr(o["B"]), nat.ptr(o["C"]), o["m"], o["n"], o["k"],
+                      o["lda"], o["ldb"], o["ldc"], al.real, al.imag, o.get("flags", 0), 0)
+        dtab = nat.to_device_table(tab, self.dev)
+        nat.check(self.lib.mpsx_gemm(self.stream, tab.ctypes.data, dtab.data_ptr(), len(ops)))
+        self._last_gemm_table = dtab
+
+    # ------------------------------------------------------------ scale
+    def scale(self, tensors: Sequence[object], factor: complex, conj: bool = False) -> None:
+        if not tensors:
+            return
+        tab = np.zeros(len(tensors), dtype=nat.SCALE_OP)
+        f = complex(factor)
+        for i, t in enumerate(tensors):
+            tab[i] = (nat.ptr(t), t.numel(), f.real, f.imag, 1 if conj else 0, 0)
+        dtab = nat.to_device_table(tab, self.dev)
+        nat.check(self.lib.mpsx_scale(self.stream, tab.ctypes.data, dtab.data_ptr(), len(tensors)))
+        self._last_scale_table = dtab

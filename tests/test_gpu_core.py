@@ -1,0 +1,307 @@
+"""GPU parity tests of the MPS API (hot path through libmpsx).
+
+Known answers restated from the reference's core_test.py (file:line cited per
+test); random circuits checked against the FP64 state-vector oracle.
+Tolerances: untruncated wavefunctions <= 1e-5 max-abs vs FP64 (complex64
+storage). Closed forms use close(): np.allclose with atol=1e-6 instead of
+numpy's default 1e-8, because the reference reaches its known answers in
+complex128 (its float64 CNOT/SWAP promote every tensor) while this path
+stores complex64, whose rounding is ~6e-8 relative.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import statevector as sv
+from oracle.mps_oracle import OracleMPS
+
+
+def _mpsim():
+    import mpsim_amd
+    from mpsim_amd import _native
+    _native.load()  # fail loudly if the library is missing
+    return mpsim_amd
+
+
+def close(a, b):
+    return np.allclose(a, b, rtol=1e-5, atol=1e-6)
+
+
+def _basis(n, bits):
+    v = np.zeros(2 ** n)
+    v[int("".join(str(b) for b in bits), 2)] = 1.0
+    return v
+
+
+def test_library_is_native():
+    m = _mpsim()
+    from mpsim_amd import _native
+    lib = _native.load()
+    import ctypes
+    arch = ctypes.create_string_buffer(64)
+    cus = ctypes.c_int(0)
+    _native.check(lib.mpsx_device_info(0, arch, 64, ctypes.byref(cus)))
+    assert arch.value.decode().startswith("gfx950"), arch.value
+    assert cus.value > 0
+
+
+def test_product_state_wavefunction():
+    m = _mpsim()
+    for n in range(2, 7):
+        for d in (2, 3, 4):
+            mps = m.MPS(n, d)
+            wf = mps.wavefunction()
+            ref = np.zeros(d ** n)
+            ref[0] = 1
+            assert close(wf, ref)  # core_test.py:377-386
+
+
+def test_one_qubit_gates():
+    """core_test.py:389-406."""
+    m = _mpsim()
+    from mpsim_amd.gates import one_state, plus_state, zero_state
+    for gate, expected in ((m.xgate(), one_state), (m.hgate(), plus_state),
+                           (m.zgate(), zero_state), (m.igate(), zero_state)):
+        for n in range(2, 6):
+            for j in range(n):
+                mps = m.MPS(n)
+                mps.apply_one_qudit_gate(gate, j)
+                assert close(mps.get_node(j).tensor.reshape(2), expected)
+
+
+def test_hadamard_all():
+    """core_test.py:421-430."""
+    m = _mpsim()
+    n = 5
+    mps = m.MPS(n)
+    mps.apply_one_qudit_gate_to_all(m.hgate())
+    assert close(mps.wavefunction(), np.ones(2 ** n) / 2 ** (n / 2))
+
+
+def test_cnot_truth_table_and_flip():
+    """core_test.py:409-472."""
+    m = _mpsim()
+    for x0, x1 in ((0, 0), (0, 1), (1, 0), (1, 1)):
+        mps = m.MPS(2)
+        if x0:
+            mps.x(0)
+        if x1:
+            mps.x(1)
+        mps.apply_two_qudit_gate(m.cnot(), 0, 1)
+        assert close(mps.wavefunction(), _basis(2, (x0, x1 ^ x0)))
+        mps = m.MPS(2)
+        if x0:
+            mps.x(0)
+        if x1:
+            mps.x(1)
+        mps.cnot(1, 0)
+        assert close(mps.wavefunction(), _basis(2, (x0 ^ x1, x1)))
+
+
+@pytest.mark.parametrize("left", [True, False])
+def test_cnot_five_qubits_all_pairs(left):
+    """core_test.py:514-535."""
+    m = _mpsim()
+    n = 5
+    for a in range(n - 1):
+        mps = m.MPS(n)
+        mps.x(a)
+        mps.cnot(a, a + 1, keep_left_canonical=left)
+        bits = [0] * n
+        bits[a] = bits[a + 1] = 1
+        assert close(mps.wavefunction(), _basis(n, bits))
+
+
+@pytest.mark.parametrize("left", [True, False])
+def test_bell_state(left):
+    """core_test.py:773-782."""
+    m = _mpsim()
+    mps = m.MPS(2)
+    mps.h(0)
+    mps.cnot(0, 1, keep_left_canonical=left)
+    assert close(mps.wavefunction(), np.array([1, 0, 0, 1]) / np.sqrt(2))
+    assert mps.bond_dimensions() == [2]
+
+
+def test_bell_truncation_known_answers():
+    """core_test.py:905-929, 1084-1101, 1123-1174; README.md:45-57."""
+    m = _mpsim()
+    mps = m.MPS(2)
+    mps.h(0)
+    mps.cnot(0, 1, maxsvals=1)
+    assert close(mps.wavefunction(), [1 / np.sqrt(2), 0, 0, 0])
+    assert np.isclose(mps.norm(), 1 / np.sqrt(2))
+    mps.renormalize()
+    assert close(mps.wavefunction(), [1, 0, 0, 0])
+    assert np.isclose(mps.norm(), 1.0)
+    for to in np.linspace(0.1, 2.0, 7):
+        mps.renormalize(to_norm=to)
+        assert close(mps.wavefunction(), [to, 0, 0, 0])
+    # fraction
+    mps = m.MPS(2)
+    mps.h(0)
+    mps.cnot(0, 1, fraction=0.5)
+    assert close(mps.wavefunction(), [1 / np.sqrt(2), 0, 0, 0])
+    mps = m.MPS(2)
+    mps.h(0)
+    mps.cnot(0, 1, fraction=1)
+    assert close(mps.wavefunction(), [1 / np.sqrt(2), 0, 0, 1 / np.sqrt(2)])
+    # maxsvals = 0 -> zero norm, renormalize raises
+    mps = m.MPS(2)
+    mps.h(0)
+    mps.cnot(0, 1, maxsvals=0)
+    assert mps.norm() == 0
+    with pytest.raises(ValueError):
+        mps.renormalize()
+
+
+def test_bond_dims_keep_zero_singular_values():
+    """core_test.py:932-973: zero singular values are kept."""
+    m = _mpsim()
+    mps = m.MPS(2)
+    mps.h(0)
+    mps.cnot(0, 1)
+    assert mps.bond_dimension_of(0) == 2
+    mps.cnot(0, 1)
+    assert mps.bond_dimension_of(0) == 2
+    mps = m.MPS(4)
+    mps.r(-1)
+    mps.apply_two_qudit_gate(m.cnot(), 0, 1, fraction=1)
+    assert mps.bond_dimensions() == [2, 1, 1]
+    mps = m.MPS(4)
+    mps.r(-1)
+    mps.apply_two_qudit_gate(m.cnot(), 0, 1, fraction=0.5)
+    assert mps.bond_dimensions() == [1, 1, 1]
+
+
+@pytest.mark.parametrize("left", [True, False])
+def test_three_cnots_is_swap(left):
+    """core_test.py:854-873."""
+    m = _mpsim()
+    for n in range(2, 8):
+        mps = m.MPS(n)
+        mps.x(0)
+        mps.cnot(0, 1, keep_left_canonical=left)
+        mps.h(-1)
+        mps.cnot(0, 1, keep_left_canonical=left)
+        mps.h(-1)
+        mps.cnot(0, 1)
+        correct = np.zeros(2 ** n)
+        correct[2 ** (n - 2)] = 1
+        assert close(mps.wavefunction(), correct)
+
+
+def test_swap_network_ghz_and_qft():
+    """core_test.py:1225-1258: non-local gates through the SWAP network."""
+    m = _mpsim()
+    for n in range(3, 8):
+        mps = m.MPS(n)
+        mps.h(0)
+        for i in range(1, n):
+            mps.cnot(0, i)
+        correct = np.zeros(2 ** n)
+        correct[0] = correct[-1] = 1 / np.sqrt(2)
+        assert close(mps.wavefunction(), correct)
+    for n in range(3, 7):
+        mps = m.MPS(n)
+        for i in range(n - 1, -1, -1):
+            mps.h(i)
+            for j in range(i - 1, -1, -1):
+                mps.apply_two_qudit_gate(m.cphase(2 ** (j - i)), j, i)
+        assert close(mps.wavefunction(), np.ones(2 ** n) / 2 ** (n / 2))
+
+
+def test_move_node_right_apply_move_back():
+    """core_test.py:750-770."""
+    m = _mpsim()
+    for n in range(3, 8):
+        mps = m.MPS(n)
+        mps.x(0)
+        mps.move_node_from_left_to_right(0, n - 2)
+        mps.cnot(n - 2, n - 1)
+        mps.move_node_from_right_to_left(n - 2, 0)
+        correct = np.zeros(2 ** n)
+        correct[2 ** (n - 1) + 1] = 1
+        assert close(mps.wavefunction(), correct)
+
+
+def test_inner_product_and_norm():
+    """core_test.py:976-1045 (inner products against dense vectors)."""
+    m = _mpsim()
+    rng = np.random.RandomState(7)
+    n = 5
+    a, b = m.MPS(n), m.MPS(n)
+    ops_a, ops_b = [], []
+    for mps, ops in ((a, ops_a), (b, ops_b)):
+        for layer in range(3):
+            for q in range(n):
+                u = m.gates.haar_unitary(2, rng)
+                ops.append(((q,), u))
+                mps.apply_one_qudit_gate(u, q)
+            for i in range(layer % 2, n - 1, 2):
+                u = m.gates.haar_unitary(4, rng)
+                ops.append(((i, i + 1), u))
+                mps.apply_two_qudit_gate(u.reshape(2, 2, 2, 2), i, i + 1)
+    wa, wb = sv.run(ops_a, n), sv.run(ops_b, n)
+    assert np.isclose(a.inner_product(b), np.inner(wa, wb.conj()), atol=1e-5)
+    assert np.isclose(b.inner_product(a), np.inner(wb, wa.conj()), atol=1e-5)
+    assert np.isclose(a.norm(), 1.0, atol=1e-5)
+
+
+@pytest.mark.parametrize("n,depth,seed", [(6, 6, 11), (8, 8, 12), (10, 6, 13)])
+def test_random_circuits_vs_statevector(n, depth, seed):
+    """simulator_test.py:274-305 restated without cirq: seeded random
+    brickwork circuits, untruncated, against the FP64 state vector."""
+    m = _mpsim()
+    from mpsim_amd.circuits import random_pairs_circuit
+    circ = random_pairs_circuit(seed, n, depth, maxsvals=10 ** 9)
+    mps = m.MPS(n)
+    for qudits, mat in circ.ops:
+        if len(qudits) == 1:
+            mps.apply_one_qudit_gate(mat, qudits[0])
+        else:
+            mps.apply_two_qudit_gate(mat.reshape(2, 2, 2, 2), *qudits)
+    ref = sv.run(circ.ops, n)
+    err = np.max(np.abs(mps.wavefunction() - ref))
+    assert err <= 1e-5, err
+
+
+def test_plain_config_16_qubits_vs_statevector():
+    """The CPU-runnable plain config: 16 qubits, brickwork depth 8 of U(2)
+    layers and CNOTs, no truncation; <= 1e-5 max-abs vs FP64."""
+    m = _mpsim()
+    from mpsim_amd.circuits import brickwork_cnot_u2
+    circ = brickwork_cnot_u2(16, 8, seed=1)
+    mps = m.MPS(16)
+    for qudits, mat in circ.ops:
+        if len(qudits) == 1:
+            mps.apply_one_qudit_gate(mat, qudits[0])
+        else:
+            mps.apply_two_qudit_gate(mat.reshape(2, 2, 2, 2), *qudits)
+    ref = sv.run(circ.ops, 16)
+    err = np.max(np.abs(mps.wavefunction() - ref))
+    assert err <= 1e-5, err
+    assert max(mps.bond_dimensions()) == 256
+
+
+@pytest.mark.parametrize("left", [True, False])
+def test_truncated_spectra_vs_fp64_oracle(left):
+    """Truncated brickwork (scaled-down single-large-circuit config): the
+    state matches the FP64 restatement gauge-invariantly (overlap) and the
+    discarded weights agree."""
+    m = _mpsim()
+    from mpsim_amd.circuits import brickwork_haar
+    from oracle.mps_oracle import gauge_invariant_error
+    circ = brickwork_haar(12, 8, 2, maxsvals=8, seed=21)
+    ora = OracleMPS(12, 2, dtype_mode="c128")
+    mps = m.MPS(12)
+    for qudits, mat in circ.ops:
+        ora.apply_two_qudit_gate(mat.reshape(2, 2, 2, 2), *qudits, maxsvals=8,
+                                 keep_left_canonical=left)
+        mps.apply_two_qudit_gate(mat.reshape(2, 2, 2, 2), *qudits, maxsvals=8,
+                                 keep_left_canonical=left)
+    assert mps.bond_dimensions() == ora.bond_dimensions()
+    err, nr, nx = gauge_invariant_error(ora.tensors, [t.cpu().numpy() for t in mps.tensors])
+    assert err <= 1e-4, err
+    assert abs(nr - nx) <= 1e-4 * nr
