@@ -60,7 +60,7 @@ class NotConvergedWarning(RuntimeWarning):
 
 EXPORTS = ("mpsx_version", "mpsx_last_error", "mpsx_device_info", "mpsx_sizeof",
            "mpsx_apply_one_site", "mpsx_two_site_plan", "mpsx_two_site", "mpsx_gemm",
-           "mpsx_scale")
+           "mpsx_scale", "mpsx_profile_enable", "mpsx_profile_read")
 
 _lock = threading.Lock()
 _lib: Optional[ctypes.CDLL] = None
@@ -88,8 +88,12 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
                                       ctypes.POINTER(i32)]
         lib.mpsx_gemm.argtypes = [vp, vp, vp, i32]
         lib.mpsx_scale.argtypes = [vp, vp, vp, i32]
+        lib.mpsx_profile_enable.argtypes = [i32]
+        lib.mpsx_profile_read.argtypes = [ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_int64), i32, i32]
         for name in ("mpsx_apply_one_site", "mpsx_two_site_plan", "mpsx_two_site", "mpsx_gemm",
-                     "mpsx_scale", "mpsx_device_info"):
+                     "mpsx_scale", "mpsx_device_info", "mpsx_profile_enable",
+                     "mpsx_profile_read"):
             getattr(lib, name).restype = i32
         sizes = [lib.mpsx_sizeof(i) for i in range(6)]
         want = [SITE_OP.itemsize, TWO_SITE_OP.itemsize, GEMM_OP.itemsize, SCALE_OP.itemsize,
@@ -141,3 +145,26 @@ def ptr(t) -> int:
     if not t.is_contiguous():
         raise ValueError("libmpsx needs contiguous tensors")
     return int(t.data_ptr())
+
+
+# --------------------------------------------------------------------- timing
+def profile_enable(on: bool) -> None:
+    check(load().mpsx_profile_enable(1 if on else 0))
+
+
+def profile_read(reset: bool = True) -> dict:
+    """{kernel name: (total ms, launches)} since the last reset; synchronises
+    the recorded HIP events."""
+    lib = load()
+    cap, nlen = 64, 64
+    names = ctypes.create_string_buffer(cap * nlen)
+    tot = (ctypes.c_double * cap)()
+    cnt = (ctypes.c_int64 * cap)()
+    n = lib.mpsx_profile_read(names, nlen, tot, cnt, cap, 1 if reset else 0)
+    if n < 0:
+        check(n)
+    out = {}
+    for i in range(min(n, cap)):
+        nm = names.raw[i * nlen:(i + 1) * nlen].split(b"\0", 1)[0].decode()
+        out[nm] = (float(tot[i]), int(cnt[i]))
+    return out

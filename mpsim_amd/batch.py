@@ -1,0 +1,130 @@
+"""Layer-batched execution of many circuits on one GPU.
+
+The reference applies one gate at a time (core.py:1245, "TODO:
+Parallelize"). Here every circuit advances one layer per tick, and all the
+one-site gates of a tick (over all circuits) go to one grouped launch and
+all the two-site updates of a tick (disjoint pairs of every circuit's
+current layer) go to one grouped theta + Jacobi SVD launch sequence. Gate
+matrices are uploaded once per circuit and stay resident.
+
+`run_plain` is the one-gate-at-a-time path through the public MPS API, the
+baseline the batched form is compared with.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from .circuits import Circuit
+from .core import MPS
+from .engine import Engine, SvdConfig, TwoSiteItem
+
+MAX_GROUP = 65535
+
+
+@dataclass
+class DeviceCircuit:
+    """A circuit whose gates are resident on the device, with its MPS."""
+    circuit: Circuit
+    mps: MPS
+    gates: List[List[object]]          # per layer, per op: device matrix
+    t: int = 0                         # next layer
+    discarded: List[object] = field(default_factory=list)
+
+    @property
+    def done(self) -> bool:
+        return self.t >= len(self.circuit.layers)
+
+
+def upload_gates(engine: Engine, circuit: Circuit) -> List[List[object]]:
+    """One host->device copy of every gate matrix of the circuit."""
+    torch = engine.torch
+    mats = [np.asarray(m, dtype=np.complex64).ravel() for layer in circuit.layers for _, m in layer]
+    if not mats:
+        return [[] for _ in circuit.layers]
+    flat = torch.from_numpy(np.concatenate(mats)).to(engine.dev)
+    out, pos = [], 0
+    for layer in circuit.layers:
+        row = []
+        for q, m in layer:
+            dim = circuit.d ** len(q)
+            row.append(flat[pos:pos + dim * dim].view(dim, dim))
+            pos += dim * dim
+        out.append(row)
+    return out
+
+
+def prepare(circuit: Circuit, engine: Optional[Engine] = None) -> DeviceCircuit:
+    eng = engine or Engine.get()
+    mps = MPS(circuit.n, circuit.d, device=eng.dev)
+    return DeviceCircuit(circuit, mps, upload_gates(eng, circuit))
+
+
+def reset(dc: DeviceCircuit) -> None:
+    """Back to |0...0> (gates stay resident)."""
+    dc.mps = MPS(dc.circuit.n, dc.circuit.d, device=dc.mps.device)
+    dc.t = 0
+    dc.discarded = []
+
+
+def tick(engine: Engine, dcs: Sequence[DeviceCircuit], svd: Optional[SvdConfig] = None,
+         keep_discarded: bool = False) -> int:
+    """Advances every unfinished circuit by one layer; returns the number of
+    two-site updates performed."""
+    one, two, owners = [], [], []
+    for dc in dcs:
+        if dc.done:
+            continue
+        c = dc.circuit
+        layer, glayer = c.layers[dc.t], dc.gates[dc.t]
+        tensors = dc.mps._tensors
+        d = c.d
+        for (q, _), g in zip(layer, glayer):
+            if len(q) == 1:
+                t = tensors[q[0]]
+                if t.numel():
+                    one.append((t, t, g))
+            else:
+                i, j = q
+                if j != i + 1:
+                    raise ValueError("batched execution takes nearest-neighbour pairs (i, i+1)")
+                a, b = tensors[i], tensors[i + 1]
+                M, N = d * a.shape[0], d * b.shape[2]
+                k = min(M, N) if c.maxsvals is None else min(M, N, int(c.maxsvals))
+                two.append(TwoSiteItem(a, b, g, k))
+                owners.append((dc, i))
+        dc.t += 1
+    for s in range(0, len(one), MAX_GROUP):
+        engine.one_site(one[s:s + MAX_GROUP])
+    if two:
+        res = engine.two_site(two, svd)
+        for (dc, i), r in zip(owners, res):
+            dc.mps._tensors[i] = r.a_out
+            dc.mps._tensors[i + 1] = r.b_out
+            if keep_discarded:
+                dc.discarded.append(r.discarded)
+    return len(two)
+
+
+def run_batched(dcs: Sequence[DeviceCircuit], engine: Optional[Engine] = None,
+                svd: Optional[SvdConfig] = None) -> int:
+    eng = engine or Engine.get()
+    total = 0
+    while any(not dc.done for dc in dcs):
+        total += tick(eng, dcs, svd)
+    return total
+
+
+def run_plain(circuit: Circuit, device=None) -> MPS:
+    """One gate per call through the public API (the unbatched baseline)."""
+    mps = MPS(circuit.n, circuit.d, device=device)
+    d = circuit.d
+    kw = {} if circuit.maxsvals is None else {"maxsvals": int(circuit.maxsvals)}
+    for q, m in circuit.ops:
+        if len(q) == 1:
+            mps.apply_one_qudit_gate(m, q[0])
+        else:
+            mps.apply_two_qudit_gate(np.asarray(m).reshape([d] * 4), q[0], q[1], **kw)
+    return mps

@@ -6,7 +6,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "common.h"
 #include "launch.h"
@@ -57,6 +60,55 @@ Resolved resolve(const mpsx_svd_opts *o) {
         if (o->tol > 0) r.tol = o->tol;
     }
     return r;
+}
+
+// ------------------------------------------------------------ profiler
+struct ProfRec {
+    const char *name;
+    hipEvent_t a, b;
+};
+struct Profiler {
+    std::mutex mu;
+    bool on = false;
+    std::vector<ProfRec> recs;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+};
+Profiler &prof() {
+    static Profiler p;
+    return p;
+}
+
+// Launches `fn` and, when profiling is on, brackets it with events.
+template <class F>
+hipError_t timed(const char *name, hipStream_t s, F &&fn) {
+    Profiler &p = prof();
+    if (!p.on) return fn();
+    hipEvent_t a, b;
+    {
+        std::lock_guard<std::mutex> g(p.mu);
+        a = p.get();
+        b = p.get();
+    }
+    if (!a || !b) return fn();
+    hipError_t e = hipEventRecord(a, s);
+    if (e != hipSuccess) return e;
+    e = fn();
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(b, s);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> g(p.mu);
+    p.recs.push_back({name, a, b});
+    return hipSuccess;
 }
 
 int *poll_buffer() {
@@ -120,7 +172,7 @@ int mpsx_apply_one_site(mpsx_stream stream, const mpsx_site_op *h, const mpsx_si
         max_d = std::max(max_d, o.d);
     }
     if (max_pairs == 0) return MPSX_OK;
-    HIP_TRY(launch_one_site((hipStream_t)stream, d, count, max_pairs, max_d), "k_one_site");
+    HIP_TRY(timed("k_one_site", (hipStream_t)stream, [&] { return launch_one_site((hipStream_t)stream, d, count, max_pairs, max_d); }), "k_one_site");
     return MPSX_OK;
 }
 
@@ -137,7 +189,7 @@ int mpsx_scale(mpsx_stream stream, const mpsx_scale_op *h, const mpsx_scale_op *
         max_n = std::max(max_n, h[i].n);
     }
     if (max_n == 0) return MPSX_OK;
-    HIP_TRY(launch_scale((hipStream_t)stream, d, count, max_n), "k_scale");
+    HIP_TRY(timed("k_scale", (hipStream_t)stream, [&] { return launch_scale((hipStream_t)stream, d, count, max_n); }), "k_scale");
     return MPSX_OK;
 }
 
@@ -164,7 +216,7 @@ int mpsx_gemm(mpsx_stream stream, const mpsx_gemm_op *h, const mpsx_gemm_op *d, 
     }
     if (max_m == 0 || max_n == 0) return MPSX_OK;
     if ((max_m + 31) / 32 > 65535) return fail(MPSX_EINVAL, "m too large for one launch");
-    HIP_TRY(launch_gemm((hipStream_t)stream, d, count, max_m, max_n), "k_gemm");
+    HIP_TRY(timed("k_gemm", (hipStream_t)stream, [&] { return launch_gemm((hipStream_t)stream, d, count, max_m, max_n); }), "k_gemm");
     return MPSX_OK;
 }
 
@@ -268,12 +320,12 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
     HIP_TRY(hipStreamIsCapturing(stream, &cap), "hipStreamIsCapturing");
     const bool can_poll = r.check_every > 0 && cap == hipStreamCaptureStatusNone;
 
-    if (rows > 0) HIP_TRY(launch_init_rows(stream, d, count, ws, rows), "k_init_rows");
+    if (rows > 0) HIP_TRY(timed("k_init_rows", stream, [&] { return launch_init_rows(stream, d, count, ws, rows); }), "k_init_rows");
     for (int D = 2; D <= 4; ++D)
-        if (has_d[D] && tiles > 0) HIP_TRY(launch_theta(stream, D, d, count, ws, tiles), "k_theta");
+        if (has_d[D] && tiles > 0) HIP_TRY(timed("k_theta", stream, [&] { return launch_theta(stream, D, d, count, ws, tiles); }), "k_theta");
     if (rows > 0) {
-        HIP_TRY(launch_rownorm(stream, d, count, ws, rows), "k_rownorm");
-        HIP_TRY(launch_fro(stream, d, count, ws, kNullRel), "k_fro");
+        HIP_TRY(timed("k_rownorm", stream, [&] { return launch_rownorm(stream, d, count, ws, rows); }), "k_rownorm");
+        HIP_TRY(timed("k_fro", stream, [&] { return launch_fro(stream, d, count, ws, kNullRel); }), "k_fro");
     }
 
     int sweeps = 0;
@@ -284,10 +336,13 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
             for (int round = 0; round < max_nb - 1; ++round)
                 for (int b : {4, 8, 16})
                     if (has_b[b])
-                        HIP_TRY(launch_jacobi_round(stream, b, d, count, ws, pairs, round, r.tol,
-                                                    r.tol * 0.01, r.inner_sweeps),
+                        HIP_TRY(timed("k_jacobi_round", stream,
+                                      [&] {
+                                          return launch_jacobi_round(stream, b, d, count, ws, pairs, round,
+                                                                     r.tol, r.tol * 0.01, r.inner_sweeps);
+                                      }),
                                 "k_jacobi_round");
-            HIP_TRY(launch_sweep_end(stream, d, count, ws, s), "k_sweep_end");
+            HIP_TRY(timed("k_sweep_end", stream, [&] { return launch_sweep_end(stream, d, count, ws, s); }), "k_sweep_end");
             ++sweeps;
             if (pin && ((s + 1) % r.check_every == 0 || s + 1 == r.max_sweeps)) {
                 const TwoSiteHeader *hdr = reinterpret_cast<const TwoSiteHeader *>(ws);
@@ -306,14 +361,56 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
     }
 
     if (rows > 0) {
-        HIP_TRY(launch_rownorm(stream, d, count, ws, rows), "k_rownorm");
-        HIP_TRY(launch_sort(stream, d, count, ws), "k_sort");
+        HIP_TRY(timed("k_rownorm", stream, [&] { return launch_rownorm(stream, d, count, ws, rows); }), "k_rownorm");
+        HIP_TRY(timed("k_sort", stream, [&] { return launch_sort(stream, d, count, ws); }), "k_sort");
     }
-    if (outs > 0) HIP_TRY(launch_gather(stream, d, count, ws, outs), "k_gather");
+    if (outs > 0) HIP_TRY(timed("k_gather", stream, [&] { return launch_gather(stream, d, count, ws, outs); }), "k_gather");
     if (sweeps_out) *sweeps_out = sweeps;
     if (can_poll && !converged)
         return fail(MPSX_ENOTCONV, "Jacobi SVD not converged after %d sweeps", sweeps);
     return MPSX_OK;
+}
+
+int mpsx_profile_enable(int on) {
+    Profiler &p = prof();
+    std::lock_guard<std::mutex> g(p.mu);
+    p.on = on != 0;
+    return MPSX_OK;
+}
+
+int mpsx_profile_read(char *names, int name_len, double *total_ms, int64_t *launches, int cap,
+                      int reset) {
+    Profiler &p = prof();
+    std::lock_guard<std::mutex> g(p.mu);
+    std::map<std::string, std::pair<double, int64_t>> agg;
+    for (const ProfRec &r : p.recs) {
+        HIP_TRY(hipEventSynchronize(r.b), "hipEventSynchronize");
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
+        auto &slot = agg[r.name];
+        slot.first += ms;
+        slot.second += 1;
+    }
+    int i = 0;
+    for (const auto &kv : agg) {
+        if (i < cap) {
+            if (names && name_len > 0) {
+                std::strncpy(names + (size_t)i * name_len, kv.first.c_str(), (size_t)name_len - 1);
+                names[(size_t)i * name_len + name_len - 1] = '\0';
+            }
+            if (total_ms) total_ms[i] = kv.second.first;
+            if (launches) launches[i] = kv.second.second;
+        }
+        ++i;
+    }
+    if (reset) {
+        for (const ProfRec &r : p.recs) {
+            p.pool.push_back(r.a);
+            p.pool.push_back(r.b);
+        }
+        p.recs.clear();
+    }
+    return i;
 }
 
 }  // extern "C"

@@ -126,7 +126,9 @@ def is_projector(gate) -> bool:
 # ------------------------------------------------------------------ states
 zero_state = np.array([1.0, 0.0], dtype=np.complex64)
 one_state = np.array([0.0, 1.0], dtype=np.complex64)
-plus_state = 1.0 / np.sqrt(2) * (zero_state + one_state)
+# gates.py:74 under numpy 1.x casting rules (complex64); numpy>=2 would
+# promote the float64 scalar product to complex128.
+plus_state = ((zero_state + one_state) / np.sqrt(2)).astype(np.complex64)
 
 
 def computational_basis_state(state: int, dim: int = 2) -> Node:
@@ -143,7 +145,7 @@ def computational_basis_state(state: int, dim: int = 2) -> Node:
 
 
 # ------------------------------------------------------------------ gates
-_hmatrix = 1 / np.sqrt(2) * np.array([[1.0, 1.0], [1.0, -1.0]], dtype=np.complex64)
+_hmatrix = (np.array([[1.0, 1.0], [1.0, -1.0]]) / np.sqrt(2)).astype(np.complex64)
 _imatrix = np.array([[1.0, 0.0], [0.0, 1.0]], dtype=np.complex64)
 _xmatrix = np.array([[0.0, 1.0], [1.0, 0.0]], dtype=np.complex64)
 _ymatrix = np.array([[0.0, -1j], [1j, 0.0]], dtype=np.complex64)

@@ -30,7 +30,9 @@ import numpy as np
 # ---------------------------------------------------------------------------
 
 # gates.py:104-110: one-qubit gate matrices are complex64.
-_H = (1 / np.sqrt(2) * np.array([[1.0, 1.0], [1.0, -1.0]], dtype=np.complex64))
+# numpy>=2 would promote float64-scalar * complex64 to complex128; the
+# reference (numpy 1.x value-based casting) gets complex64.
+_H = (np.array([[1.0, 1.0], [1.0, -1.0]]) / np.sqrt(2)).astype(np.complex64)
 _I = np.array([[1.0, 0.0], [0.0, 1.0]], dtype=np.complex64)
 _X = np.array([[0.0, 1.0], [1.0, 0.0]], dtype=np.complex64)
 _Y = np.array([[0.0, -1j], [1j, 0.0]], dtype=np.complex64)

@@ -1,0 +1,177 @@
+"""CPU tests of the C ABI surface and the host-side logic (no GPU needed).
+
+The library is loaded and every symbol include/mpsx.h declares is resolved;
+the planning call (pure host code) is exercised; nothing launches a kernel.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "mpsx.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(mpsx_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from mpsim_amd import _native as nat
+    lib = nat.load()
+    syms = header_symbols()
+    assert len(syms) >= 10
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(nat.EXPORTS) <= set(syms)
+    assert lib.mpsx_version() >= 100
+
+
+def test_struct_layouts_match():
+    from mpsim_amd import _native as nat
+    lib = nat.load()
+    assert lib.mpsx_sizeof(1) == nat.TWO_SITE_OP.itemsize == 144
+    assert lib.mpsx_sizeof(0) == nat.SITE_OP.itemsize
+    assert lib.mpsx_sizeof(2) == nat.GEMM_OP.itemsize
+    assert lib.mpsx_sizeof(3) == nat.SCALE_OP.itemsize
+    assert lib.mpsx_sizeof(4) == ctypes.sizeof(nat.SvdOpts)
+    assert lib.mpsx_sizeof(9) == 0
+
+
+def _table(rows):
+    from mpsim_amd import _native as nat
+    tab = np.zeros(len(rows), dtype=nat.TWO_SITE_OP)
+    for i, (cl, cm, cr, d, k, fl) in enumerate(rows):
+        for f in ("a", "b", "gate", "a_out", "b_out"):
+            tab[i][f] = 0x1000
+        tab[i]["chi_l"], tab[i]["chi_m"], tab[i]["chi_r"] = cl, cm, cr
+        tab[i]["d"], tab[i]["k"], tab[i]["flags"] = d, k, fl
+    return tab
+
+
+def _plan(tab, opts=None):
+    from mpsim_amd import _native as nat
+    lib = nat.load()
+    o = opts or nat.SvdOpts(0, 0, 1, 0, 0.0)
+    nb = ctypes.c_size_t(0)
+    rc = lib.mpsx_two_site_plan(tab.ctypes.data, len(tab), ctypes.byref(o), ctypes.byref(nb))
+    return rc, nb.value, lib.mpsx_last_error().decode()
+
+
+def test_plan_layout_and_prefixes():
+    tab = _table([(1, 1, 1, 2, 2, 0), (256, 256, 256, 2, 256, 0), (4, 8, 2, 3, 6, 1)])
+    rc, nbytes, _ = _plan(tab)
+    assert rc == 0
+    p = tab
+    # orientation: absorb-left problems orthogonalise theta^T's rows
+    assert (p[0]["C"], p[0]["R"]) == (2, 2)
+    assert (p[1]["C"], p[1]["R"]) == (512, 512)
+    assert (p[2]["C"], p[2]["R"]) == (6, 12)
+    for r in p:
+        assert r["nb"] % 2 == 0 and r["Cp"] == r["nb"] * r["bw"] and r["Cp"] >= r["C"]
+        assert r["LD"] >= r["R"] + r["Cp"] and r["ws_off"] % 256 == 0
+    assert list(p["tile_base"]) == [0, 1, 65]
+    assert list(p["row_base"]) == [0, int(p[0]["Cp"]), int(p[0]["Cp"] + p[1]["Cp"])]
+    assert list(p["out_base"]) == [0, 4, 4 + 512 + 256]
+    assert nbytes > p[2]["ws_off"]
+
+
+@pytest.mark.parametrize("row,msg", [
+    ((4, 4, 4, 2, 9, 0), "k=9"),
+    ((4, 4, 4, 5, 2, 0), "d=5"),
+    ((-1, 4, 4, 2, 2, 0), "negative"),
+])
+def test_plan_rejects_bad_descriptors(row, msg):
+    rc, _, err = _plan(_table([row]))
+    assert rc == 1 and msg in err
+
+
+def test_plan_rejects_bad_options():
+    from mpsim_amd import _native as nat
+    rc, _, err = _plan(_table([(4, 4, 4, 2, 8, 0)]), nat.SvdOpts(5, 0, 1, 0, 0.0))
+    assert rc == 1 and "block" in err
+
+
+def test_two_site_rejects_unplanned_table():
+    from mpsim_amd import _native as nat
+    lib = nat.load()
+    tab = _table([(4, 4, 4, 2, 8, 0)])
+    tab[0]["ws_off"] = 12345
+    o = nat.SvdOpts(0, 0, 1, 0, 0.0)
+    sweeps = ctypes.c_int(0)
+    rc = lib.mpsx_two_site(None, tab.ctypes.data, tab.ctypes.data, 1, ctypes.byref(o),
+                           ctypes.c_void_p(0x1000), 1 << 20, ctypes.byref(sweeps))
+    assert rc == 1 and "plan" in lib.mpsx_last_error().decode()
+
+
+def test_haar_generator_is_the_reference_recipe_bit_for_bit():
+    from oracle import mps_oracle as mo
+    from mpsim_amd import gates
+    for nq, d, seed in ((1, 2, 0), (2, 2, 7), (2, 3, 11)):
+        a = gates.haar_random_unitary(nq, d, seed=seed).tensor
+        b = mo.haar_random_unitary(nq, d, seed=seed)
+        assert np.array_equal(a, b)
+
+
+def test_bond_plan_matches_oracle_growth():
+    """The host-side shape plan (zero singular values kept) equals the bond
+    dimensions the oracle reaches on the same circuit."""
+    from oracle.mps_oracle import OracleMPS
+    from mpsim_amd.circuits import bond_plan, brickwork_haar, random_pairs_circuit
+    for circ in (brickwork_haar(10, 6, 2, maxsvals=8, seed=4),
+                 random_pairs_circuit(8, 9, 5, maxsvals=4)):
+        plan = bond_plan(circ)
+        m = OracleMPS(circ.n, circ.d, dtype_mode="c128")
+        for layer, bonds in zip(circ.layers, plan):
+            m.apply_ops(layer, maxsvals=circ.maxsvals)
+            assert m.bond_dimensions() == bonds
+
+
+def test_split_option_parsing():
+    """core.py:1104-1130."""
+    from mpsim_amd.core import max_bond_dimensions_for, parse_split_options
+    assert parse_split_options({}, 4) == (None, True)
+    assert parse_split_options({"maxsvals": 3.7}, 4) == (3, True)
+    assert parse_split_options({"fraction": 0.5}, 4) == (2, True)
+    assert parse_split_options({"fraction": 0.5}, 2) == (1, True)
+    assert parse_split_options({"fraction": 0.25}, 2) == (0, True)   # banker's rounding
+    assert parse_split_options({"keep_left_canonical": False}, 4) == (None, False)
+    with pytest.raises(ValueError):
+        parse_split_options({"fraction": 0.5, "maxsvals": 1}, 4)
+    with pytest.raises(ValueError):
+        parse_split_options({"fraction": -0.1}, 4)
+    assert max_bond_dimensions_for(6, 10) == [10, 100, 1000, 100, 10]
+
+
+def test_workloads_are_seeded_and_shaped():
+    from mpsim_amd.circuits import (brickwork_cnot_u2, brickwork_haar, circuit_work,
+                                    poisson_requests, uneven_batch)
+    c1 = brickwork_cnot_u2()
+    assert c1.two_site_count() == 60 and c1.maxsvals is None
+    c2 = brickwork_haar()
+    assert c2.two_site_count() == 630
+    fl, by, cnt = circuit_work(c2)
+    assert cnt == 630 and fl > 630 * 1e6
+    a, b = uneven_batch(4), uneven_batch(4)
+    assert [x.n for x in a] == [x.n for x in b]
+    assert all(12 <= x.n <= 64 and 4 <= x.depth // 2 <= 24 for x in a)
+    assert np.array_equal(a[0].ops[0][1], b[0].ops[0][1])
+    reqs = poisson_requests(duration=0.2)
+    assert all(r.arrival < 0.2 for r in reqs) and len(reqs) > 10
+
+
+def test_per_update_work_accounting():
+    """The chi=256, d=2 figures the benchmark uses (11.82 GFLOP, 4.19 MB)."""
+    from mpsim_amd.circuits import two_site_bytes, two_site_flops
+    f = two_site_flops(256, 256, 256, 2, 256)
+    assert abs(f / 1e9 - 11.82) < 0.01
+    assert abs(two_site_bytes(256, 256, 256, 2, 256) / 1e6 - 4.19) < 0.01
+
+
+def test_simulator_rejects_bad_programs():
+    from mpsim_amd.mpsim_cirq import MPSimulator
+    with pytest.raises(ValueError):
+        MPSimulator().simulate(42)
