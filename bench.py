@@ -39,7 +39,105 @@ def parse():
     ap.add_argument("--cpu-updates", type=int, default=6)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--inner", type=int, default=0)
+    ap.add_argument("--workload", default="large",
+                    choices=["large", "uneven", "served", "plain16", "qutrit"],
+                    help="large (default, the metric line) | uneven | served | plain16 | qutrit")
+    ap.add_argument("--circuits", type=int, default=512, help="uneven: circuits in the batch")
+    ap.add_argument("--plain-sample", type=int, default=16,
+                    help="uneven/served: circuits timed one at a time on the plain path")
+    ap.add_argument("--requests", type=int, default=0, help="served: first N requests (0 = all)")
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
     return ap.parse_args()
+
+
+def emit(line):
+    print(json.dumps(line), flush=True)
+
+
+def base_line(metric, value, unit, args, ms_per_step, config, dtype="c64"):
+    return {"metric": metric, "value": round(value, 4), "unit": unit, "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic (seeded Haar gates)", "config": config}
+
+
+def run_uneven(args, eng, svd):
+    """Batched uneven circuits: grouped layer ticks over the whole batch vs the
+    plain path (one circuit, one gate per call) on a sample."""
+    import torch
+    from mpsim_amd.batch import prepare, reset, run_batched, run_plain
+    from mpsim_amd.circuits import uneven_batch
+    circs = uneven_batch(args.circuits)
+    dcs = [prepare(c, eng) for c in circs]
+    updates = sum(c.two_site_count() for c in circs)
+    for _ in range(args.warmup):
+        for dc in dcs:
+            reset(dc)
+        run_batched(dcs, eng, svd)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        for dc in dcs:
+            reset(dc)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run_batched(dcs, eng, svd)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    tb = sum(times) / len(times)
+    sample = circs[:args.plain_sample]
+    run_plain(sample[0], device=eng.dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in sample:
+        run_plain(c, device=eng.dev)
+    torch.cuda.synchronize()
+    tp = (time.perf_counter() - t0) / len(sample)
+    value = len(circs) / tb
+    line = base_line("batched uneven circuits/s vs one-at-a-time GPU path", value, "circuits/s",
+                     args, 1e3 * tb, {"workload": f"{len(circs)} seeded circuits, n in [12,64], "
+                                      "depth in [4,24], chi cap in {32,64,128,256}",
+                                      "circuits": len(circs), "two_site_updates": updates})
+    line["extra"] = {"plain_gpu_circuits_per_s": round(1.0 / tp, 4),
+                     "plain_sample": f"first {len(sample)} circuits, one at a time",
+                     "batched_vs_plain": round(value * tp, 3),
+                     "updates_per_s": round(updates / tb, 2)}
+    emit(line)
+
+
+def run_served(args, eng, svd):
+    """Poisson stream, continuous batching; vs the plain path on a sample."""
+    import torch
+    from mpsim_amd.batch import run_plain
+    from mpsim_amd.circuits import poisson_requests
+    from mpsim_amd.serve import MPSServer
+    reqs = poisson_requests(args.rate, args.duration, limit=args.requests or None)
+    srv = MPSServer(eng, hbm_budget=int(args.budget_gb * (1 << 30)), svd=svd)
+    warm = poisson_requests(args.rate, 0.05)
+    srv.run(warm, realtime=False)
+    torch.cuda.synchronize()
+    rep = srv.run(reqs, realtime=True)
+    sample = [r.circuit for r in reqs[:args.plain_sample]]
+    t0 = time.perf_counter()
+    for c in sample:
+        run_plain(c, device=eng.dev)
+    torch.cuda.synchronize()
+    tp = (time.perf_counter() - t0) / max(len(sample), 1)
+    line = base_line("served circuits/s (continuous batching) with p50/p99 latency",
+                     rep.circuits_per_s, "circuits/s", args, 1e3 * rep.wall_s / max(rep.completed, 1),
+                     {"workload": f"Poisson {args.rate}/s arrivals, first {len(reqs)} requests of "
+                                  f"the {args.duration}s stream, n in [16,80], depth in [4,32], "
+                                  "maxsvals in {64,128,256,512}", "requests": len(reqs),
+                      "hbm_budget_gb": args.budget_gb})
+    line["extra"] = {"p50_latency_s": round(rep.p50_s, 4), "p99_latency_s": round(rep.p99_s, 4),
+                     "served_updates_per_s": round(rep.updates_per_s, 2),
+                     "max_active": rep.max_active, "ticks": rep.ticks, "wall_s": round(rep.wall_s, 3),
+                     "offered_circuits_per_s": args.rate,
+                     "plain_gpu_circuits_per_s": round(1.0 / tp, 4),
+                     "served_vs_plain": round(rep.circuits_per_s * tp, 3)}
+    emit(line)
 
 
 def svd_flops(circuit):
@@ -91,6 +189,10 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     eng = Engine.get("cuda:0")
     svd = SvdConfig(block=args.block, inner_sweeps=args.inner)
+    if args.workload == "uneven":
+        return run_uneven(args, eng, svd)
+    if args.workload == "served":
+        return run_served(args, eng, svd)
     circ = brickwork_haar(64, 20, 2, maxsvals=256, seed=2)
     flops, bytes_, updates = circuit_work(circ)
     svd_f, _ = svd_flops(circ)

@@ -69,10 +69,20 @@ def reset(dc: DeviceCircuit) -> None:
     dc.discarded = []
 
 
+def _ws_estimate(it: TwoSiteItem) -> int:
+    """Upper bound of one problem's Jacobi workspace bytes."""
+    cl, d, _ = it.a.shape
+    cr = it.b.shape[2]
+    C, R = (d * cr, d * cl) if it.absorb_left else (d * cl, d * cr)
+    cp = C + 31
+    return 8 * cp * (R + cp + 4) + 16 * cp + 1024
+
+
 def tick(engine: Engine, dcs: Sequence[DeviceCircuit], svd: Optional[SvdConfig] = None,
-         keep_discarded: bool = False) -> int:
+         keep_discarded: bool = False, ws_budget: int = 8 << 30) -> int:
     """Advances every unfinished circuit by one layer; returns the number of
-    two-site updates performed."""
+    two-site updates performed. The two-site updates of the tick go to the
+    library in as few calls as the workspace budget allows."""
     one, two, owners = [], [], []
     for dc in dcs:
         if dc.done:
@@ -98,13 +108,19 @@ def tick(engine: Engine, dcs: Sequence[DeviceCircuit], svd: Optional[SvdConfig] 
         dc.t += 1
     for s in range(0, len(one), MAX_GROUP):
         engine.one_site(one[s:s + MAX_GROUP])
-    if two:
-        res = engine.two_site(two, svd)
-        for (dc, i), r in zip(owners, res):
+    start = 0
+    while start < len(two):
+        end, acc = start, 0
+        while end < len(two) and (end == start or acc + _ws_estimate(two[end]) <= ws_budget):
+            acc += _ws_estimate(two[end])
+            end += 1
+        res = engine.two_site(two[start:end], svd)
+        for (dc, i), r in zip(owners[start:end], res):
             dc.mps._tensors[i] = r.a_out
             dc.mps._tensors[i + 1] = r.b_out
             if keep_discarded:
                 dc.discarded.append(r.discarded)
+        start = end
     return len(two)
 
 

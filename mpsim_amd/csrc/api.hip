@@ -40,7 +40,7 @@ int hip_fail(hipError_t e, const char *what) {
 
 constexpr int64_t kHeaderBytes = 1024;
 constexpr int kDefaultMaxSweeps = 30;
-constexpr int kDefaultInnerSweeps = 3;
+constexpr int kDefaultInnerSweeps = 1;
 constexpr double kDefaultTol = 1e-5;
 constexpr double kNullRel = 1e-6;
 constexpr int64_t kGridLimit = 2147483647LL;
@@ -51,7 +51,7 @@ struct Resolved {
 };
 
 Resolved resolve(const mpsx_svd_opts *o) {
-    Resolved r{8, kDefaultMaxSweeps, 1, kDefaultInnerSweeps, kDefaultTol};
+    Resolved r{0, kDefaultMaxSweeps, 1, kDefaultInnerSweeps, kDefaultTol};
     if (o) {
         if (o->block) r.block = o->block;
         if (o->max_sweeps) r.max_sweeps = o->max_sweeps;
@@ -110,6 +110,10 @@ hipError_t timed(const char *name, hipStream_t s, F &&fn) {
     p.recs.push_back({name, a, b});
     return hipSuccess;
 }
+
+// Rows per Jacobi block: 2b-row pivots of 16 for small problems, 32 once
+// there are enough rows to fill several pivots.
+int auto_block(int C) { return C <= 64 ? 8 : 16; }
 
 int *poll_buffer() {
     thread_local int *buf = nullptr;
@@ -226,8 +230,8 @@ int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts
     if (count < 0) return fail(MPSX_EINVAL, "count < 0");
     if (count > 0 && !h) return fail(MPSX_EINVAL, "null descriptor table");
     const Resolved r = resolve(opts);
-    if (r.block != 4 && r.block != 8 && r.block != 16)
-        return fail(MPSX_EINVAL, "block %d not in {4, 8, 16}", r.block);
+    if (r.block != 0 && r.block != 8 && r.block != 16)
+        return fail(MPSX_EINVAL, "block %d not in {0 (auto), 8, 16}", r.block);
     if (r.max_sweeps < 1 || r.max_sweeps > 64) return fail(MPSX_EINVAL, "max_sweeps outside [1,64]");
     int64_t off = kHeaderBytes, tiles = 0, pairs = 0, rows = 0, outs = 0;
     for (int i = 0; i < count; ++i) {
@@ -248,7 +252,7 @@ int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts
         mpsx_plan &p = o.plan;
         p.C = (int32_t)(left_absorb ? N : M);
         p.R = (int32_t)(left_absorb ? M : N);
-        p.b = r.block;
+        p.b = r.block ? r.block : auto_block(p.C);
         if (p.C == 0) {
             p.nb = 0;
             p.Cp = 0;
@@ -302,7 +306,7 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
         if (p.ws_off != need || p.tile_base != tiles || p.pair_base != pairs || p.row_base != rows ||
             p.out_base != outs)
             return fail(MPSX_EINVAL, "op %d: plan missing or stale (call mpsx_two_site_plan)", i);
-        if (o.d < 2 || o.d > 4 || (p.b != 4 && p.b != 8 && p.b != 16))
+        if (o.d < 2 || o.d > 4 || (p.b != 8 && p.b != 16))
             return fail(MPSX_EINVAL, "op %d: invalid plan", i);
         need += problem_bytes(p);
         tiles += ((int64_t)o.chi_l + THETA_TL - 1) / THETA_TL * (((int64_t)o.chi_r + THETA_TR - 1) / THETA_TR);
@@ -334,7 +338,7 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
         int *pin = can_poll ? poll_buffer() : nullptr;
         for (int s = 0; s < r.max_sweeps; ++s) {
             for (int round = 0; round < max_nb - 1; ++round)
-                for (int b : {4, 8, 16})
+                for (int b : {8, 16})
                     if (has_b[b])
                         HIP_TRY(timed("k_jacobi_round", stream,
                                       [&] {

@@ -235,22 +235,38 @@ k_fro(const mpsx_two_site_op *ops, int count, char *ws, double null_rel) {
 }
 
 // ---------------------------------------------------------------- Jacobi
-// One workgroup per (problem, pivot pair) of round `round`. B2 = 2b rows.
+// One workgroup (4 waves) per (problem, pivot pair) of round `round`; the
+// pivot is B2 = 2b rows of Z. Both GEMM phases run on the FP32 matrix cores
+// (v_mfma_f32_16x16x4f32) over the real embedding of the complex rows:
+//   S = [Re Xp; Im Xp]  (NS = 2*B2 real rows)
+//   Gram:   G = S S^T,  H = (G11 + G22) + i (G21 - G12)
+//   apply:  [Re Zp'; Im Zp'] = T [Re Zp; Im Zp],  T = [[Ar, -Ai], [Ai, Ar]],
+//           A = Q^H
+// MFMA 16x16x4 operand layout (wave64): lane l holds A[l%16][l/16],
+// B[l/16][l%16]; result register r of lane l is D[4*(l/16)+r][l%16].
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+MPSX_DEV mfma_f4 mfma16(float a, float b, mfma_f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
 template <int B2>
 __global__ void __launch_bounds__(256)
 k_jacobi_round(const mpsx_two_site_op *ops, int count, char *ws, int round, double tol,
                double tol_inner, int inner_sweeps) {
     constexpr int b = B2 / 2;
-    constexpr int KC = 64;                       // columns per Gram stage
-    constexpr int NBLK = (B2 / 2) * (B2 / 2);    // 2x2 Gram blocks
-    constexpr int SLICES = 256 / NBLK;           // K-split of the Gram
-    static_assert(SLICES >= 1, "pivot too large");
+    constexpr int NS = 2 * B2;          // stacked real rows
+    constexpr int NT = NS / 16;         // 16-row tiles per side
+    constexpr int KC = 64;              // columns per Gram stage
+    constexpr int TPW = NT * NT / 4;    // Gram tiles per wave
+    static_assert(NT * NT % 4 == 0, "Gram tiles must split over 4 waves");
 
-    __shared__ float2 Xs[B2][KC + 1];
-    __shared__ double2 Hp[SLICES][B2][B2];
-    __shared__ double2 H[B2][B2];
-    __shared__ double2 Q[B2][B2];
-    __shared__ float2 Qf[B2][B2];
+    __shared__ union {
+        float2 xs[B2][KC + 1];          // Gram stage
+        double g[NS][NS + 1];           // stacked Gram
+    } sm;
+    __shared__ double2 H[B2][B2 + 1];
+    __shared__ double2 Q[B2][B2 + 1];
     __shared__ int rows[B2];
     __shared__ double prm_c[b];
     __shared__ double2 prm_s[b];
@@ -268,71 +284,82 @@ k_jacobi_round(const mpsx_two_site_op *ops, int count, char *ws, int round, doub
     const int pair = (int)(blockIdx.x - p.pair_base);
     int I, J;
     rr_pair(pair, round, p.nb, &I, &J);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
     if (tid < B2) rows[tid] = tid < b ? I * b + tid : J * b + (tid - b);
     float2 *Z = z_ptr(ws, p);
     const int LD = p.LD, R = p.R;
     __syncthreads();
 
-    // ---- Gram of the pivot rows: H = Xp Xp^H
-    const int blk = tid % NBLK, slice = tid / NBLK;
-    const int bi = blk / (B2 / 2), bj = blk % (B2 / 2);
-    double2 acc64[2][2];
+    // ---- Gram G = S S^T on the matrix cores; FP64 across KC-column stages
+    double gacc[TPW][4];
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int t = 0; t < TPW; ++t)
 #pragma unroll
-        for (int v = 0; v < 2; ++v) acc64[u][v] = make_double2(0.0, 0.0);
+        for (int r = 0; r < 4; ++r) gacc[t][r] = 0.0;
+    // Stage loads are issued one stage ahead into registers (EPT per thread)
+    // so that global latency overlaps the matrix-core work of the stage.
+    constexpr int EPT = B2 * KC / 256;
+    float2 pre[EPT];
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+        const int e = tid + 256 * u, rr = e / KC, cc = e % KC;
+        pre[u] = cc < R ? Z[(int64_t)rows[rr] * LD + cc] : make_float2(0.f, 0.f);
+    }
     for (int c0 = 0; c0 < R; c0 += KC) {
-        for (int e = tid; e < B2 * KC; e += 256) {
-            const int rr = e / KC, cc = e % KC;
-            const int c = c0 + cc;
-            Xs[rr][cc] = c < R ? Z[(int64_t)rows[rr] * LD + c] : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < EPT; ++u) {
+            const int e = tid + 256 * u;
+            sm.xs[e / KC][e % KC] = pre[u];
         }
         __syncthreads();
-        if (slice < SLICES) {
-            float2 acc32[2][2];
+        if (c0 + KC < R) {
 #pragma unroll
-            for (int u = 0; u < 2; ++u)
-#pragma unroll
-                for (int v = 0; v < 2; ++v) acc32[u][v] = make_float2(0.f, 0.f);
-            for (int cc = slice; cc < KC; cc += SLICES) {
-                const float2 x0 = Xs[2 * bi][cc], x1 = Xs[2 * bi + 1][cc];
-                const float2 y0 = Xs[2 * bj][cc], y1 = Xs[2 * bj + 1][cc];
-                // H_ij += x_i * conj(x_j)
-                acc32[0][0] = cfma_conj(y0, x0, acc32[0][0]);
-                acc32[0][1] = cfma_conj(y1, x0, acc32[0][1]);
-                acc32[1][0] = cfma_conj(y0, x1, acc32[1][0]);
-                acc32[1][1] = cfma_conj(y1, x1, acc32[1][1]);
+            for (int u = 0; u < EPT; ++u) {
+                const int e = tid + 256 * u, rr = e / KC, cc = e % KC;
+                const int c = c0 + KC + cc;
+                pre[u] = c < R ? Z[(int64_t)rows[rr] * LD + c] : make_float2(0.f, 0.f);
             }
-#pragma unroll
-            for (int u = 0; u < 2; ++u)
-#pragma unroll
-                for (int v = 0; v < 2; ++v) {
-                    acc64[u][v].x += acc32[u][v].x;
-                    acc64[u][v].y += acc32[u][v].y;
-                }
         }
+        mfma_f4 acc[TPW];
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) acc[t] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+        for (int ks = 0; ks < KC / 4; ++ks) {
+            const int kk = 4 * ks + lq;
+#pragma unroll
+            for (int t = 0; t < TPW; ++t) {
+                const int tile = wave * TPW + t;
+                const int tm = tile / NT, tn = tile % NT;
+                const int ra = tm * 16 + l16, rb = tn * 16 + l16;
+                const float2 va = sm.xs[ra % B2][kk], vb = sm.xs[rb % B2][kk];
+                const float a = ra < B2 ? va.x : va.y;
+                const float bb = rb < B2 ? vb.x : vb.y;
+                acc[t] = mfma16(a, bb, acc[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < TPW; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) gacc[t][r] += (double)acc[t][r];
         __syncthreads();
     }
-    if (slice < SLICES) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
+    for (int t = 0; t < TPW; ++t) {
+        const int tile = wave * TPW + t;
+        const int tm = tile / NT, tn = tile % NT;
 #pragma unroll
-            for (int v = 0; v < 2; ++v) Hp[slice][2 * bi + u][2 * bj + v] = acc64[u][v];
+        for (int r = 0; r < 4; ++r) sm.g[tm * 16 + 4 * lq + r][tn * 16 + l16] = gacc[t][r];
     }
     __syncthreads();
     for (int e = tid; e < B2 * B2; e += 256) {
         const int i = e / B2, j = e % B2;
-        double2 s = make_double2(0.0, 0.0);
-        if (i <= j) {
-#pragma unroll
-            for (int sl = 0; sl < SLICES; ++sl) s = zadd(s, Hp[sl][i][j]);
-        } else {
-#pragma unroll
-            for (int sl = 0; sl < SLICES; ++sl) s = zadd(s, zconj(Hp[sl][j][i]));
-        }
-        if (i == j) s.y = 0.0;
-        H[i][j] = s;
+        const int u = i <= j ? i : j, v = i <= j ? j : i;  // upper triangle, mirrored
+        double re = sm.g[u][v] + sm.g[B2 + u][B2 + v];
+        double im = sm.g[B2 + u][v] - sm.g[u][B2 + v];
+        if (i > j) im = -im;
+        if (i == j) im = 0.0;
+        H[i][j] = make_double2(re, im);
         Q[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
@@ -358,6 +385,7 @@ k_jacobi_round(const mpsx_two_site_op *ops, int count, char *ws, int round, doub
         if (tid == 0) s_any = 0;
         __syncthreads();
         for (int step = 0; step < B2 - 1; ++step) {
+            int step_rot = 0;
             if (tid < b) {
                 int pp, qq;
                 rr_pair(tid, step, B2, &pp, &qq);
@@ -367,25 +395,29 @@ k_jacobi_round(const mpsx_two_site_op *ops, int count, char *ws, int round, doub
                 double cs = 1.0;
                 double2 s = make_double2(0.0, 0.0);
                 if (a > null2 && c > null2 && g > tol_inner * sqrt(a * c) && g > 0.0) {
-                    const double zeta = (c - a) / (2.0 * g);
+                    const double rg = 1.0 / g;
+                    const double zeta = 0.5 * (c - a) * rg;
                     const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                     cs = 1.0 / sqrt(1.0 + t * t);
-                    const double sn = t * cs;
-                    s = make_double2(sn * h.x / g, sn * h.y / g);  // sn * e^{i phi}
+                    const double sn = t * cs * rg;
+                    s = make_double2(sn * h.x, sn * h.y);  // sn * e^{i phi}
                     s_any = 1;
+                    step_rot = 1;  // (cs can round to exactly 1 while s != 0)
                 }
                 prm_p[tid] = pp;
                 prm_q[tid] = qq;
                 prm_c[tid] = cs;
                 prm_s[tid] = s;
             }
-            __syncthreads();
-            // H <- J^H H J on 2x2 blocks, Q <- Q J on column pairs.
+            if (!__syncthreads_or(step_rot)) continue;  // every pair already diagonal
+            // H <- J^H H J on 2x2 blocks (upper blocks, mirrored), Q <- Q J.
             // J = [[c, s], [-conj(s), c]] for each index pair (p, q).
-            constexpr int NH = b * b, NQ = B2 * b;
+            constexpr int NH = b * (b + 1) / 2, NQ = B2 * b;
             for (int it = tid; it < NH + NQ; it += 256) {
                 if (it < NH) {
-                    const int k1 = it / b, k2 = it % b;
+                    int k1 = 0, rem = it;
+                    while (rem >= b - k1) { rem -= b - k1; ++k1; }
+                    const int k2 = k1 + rem;
                     const int p1 = prm_p[k1], q1 = prm_q[k1], p2 = prm_p[k2], q2 = prm_q[k2];
                     const double c1 = prm_c[k1], c2 = prm_c[k2];
                     const double2 s1 = prm_s[k1], s2 = prm_s[k2];
@@ -397,10 +429,20 @@ k_jacobi_round(const mpsx_two_site_op *ops, int count, char *ws, int round, doub
                     const double2 m11 = zadd(zmul(h10, s2), zscale(h11, c2));
                     const double2 ns1 = make_double2(-s1.x, -s1.y);  // -s1
                     const double2 cs1 = zconj(s1);
-                    H[p1][p2] = zadd(zscale(m00, c1), zmul(ns1, m10));
-                    H[p1][q2] = zadd(zscale(m01, c1), zmul(ns1, m11));
-                    H[q1][p2] = zadd(zmul(cs1, m00), zscale(m10, c1));
-                    H[q1][q2] = zadd(zmul(cs1, m01), zscale(m11, c1));
+                    const double2 n00 = zadd(zscale(m00, c1), zmul(ns1, m10));
+                    const double2 n01 = zadd(zscale(m01, c1), zmul(ns1, m11));
+                    const double2 n10 = zadd(zmul(cs1, m00), zscale(m10, c1));
+                    const double2 n11 = zadd(zmul(cs1, m01), zscale(m11, c1));
+                    H[p1][p2] = n00;
+                    H[p1][q2] = n01;
+                    H[q1][p2] = n10;
+                    H[q1][q2] = n11;
+                    if (k1 != k2) {
+                        H[p2][p1] = zconj(n00);
+                        H[q2][p1] = zconj(n01);
+                        H[p2][q1] = zconj(n10);
+                        H[q2][q1] = zconj(n11);
+                    }
                 } else {
                     const int q = it - NH;
                     const int row = q / b, k = q % b;
@@ -417,27 +459,64 @@ k_jacobi_round(const mpsx_two_site_op *ops, int count, char *ws, int round, doub
         }
         if (!s_any) break;
     }
-    for (int e = tid; e < B2 * B2; e += 256) {
-        const int i = e / B2, j = e % B2;
-        Qf[i][j] = make_float2((float)Q[i][j].x, (float)Q[i][j].y);
-    }
-    __syncthreads();
 
-    // ---- apply: Zp <- Q^H Zp over the whole row (X and Y parts)
+    // ---- apply on the matrix cores. T[i][j] (i, j < NS) from Q:
+    //   T[i][j] = Re Q[j][i], T[i][B2+j] = Im Q[j][i],
+    //   T[B2+i][j] = -Im Q[j][i], T[B2+i][B2+j] = Re Q[j][i]   (i, j < B2)
+    // Lane l of tile row tm, k-step s needs T[16 tm + l%16][4 s + l/16].
+    float tA[NT][NS / 4];
+#pragma unroll
+    for (int tm = 0; tm < NT; ++tm)
+#pragma unroll
+        for (int s = 0; s < NS / 4; ++s) {
+            const int i = tm * 16 + l16, j = 4 * s + lq;
+            const int ii = i % B2, jj = j % B2;
+            const double2 q = Q[jj][ii];
+            float v;
+            if (i < B2)
+                v = (float)(j < B2 ? q.x : q.y);
+            else
+                v = (float)(j < B2 ? -q.y : q.x);
+            tA[tm][s] = v;
+        }
     const int W = p.R + p.Cp;
-    for (int c = tid; c < W; c += 256) {
-        float2 zout[B2];
+    int64_t roff[B2 / 4];
 #pragma unroll
-        for (int j = 0; j < B2; ++j) zout[j] = make_float2(0.f, 0.f);
-        // Q stays in LDS (broadcast reads); each thread owns column c.
-#pragma unroll 2
-        for (int i = 0; i < B2; ++i) {
-            const float2 zi = Z[(int64_t)rows[i] * LD + c];
+    for (int s = 0; s < B2 / 4; ++s) roff[s] = (int64_t)rows[4 * s + lq] * LD;
+    // lane's complex inputs of the column block: rows (4 s + lq), s < B2/4;
+    // the next block's inputs are loaded before this block's MFMAs.
+    float2 zin[B2 / 4], znx[B2 / 4];
+    {
+        const int c = wave * 16 + l16;
 #pragma unroll
-            for (int j = 0; j < B2; ++j) zout[j] = cfma_conj(Qf[i][j], zi, zout[j]);
+        for (int s = 0; s < B2 / 4; ++s) zin[s] = c < W ? Z[roff[s] + c] : make_float2(0.f, 0.f);
+    }
+    for (int c0 = wave * 16; c0 < W; c0 += 64) {
+        const int c = c0 + l16;
+        const bool ok = c < W;
+        const int cn = c + 64;
+#pragma unroll
+        for (int s = 0; s < B2 / 4; ++s) znx[s] = cn < W ? Z[roff[s] + cn] : make_float2(0.f, 0.f);
+        mfma_f4 acc[NT];
+#pragma unroll
+        for (int tm = 0; tm < NT; ++tm) acc[tm] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < NS / 4; ++s) {
+            const float bv = s < B2 / 4 ? zin[s].x : zin[s - B2 / 4].y;
+#pragma unroll
+            for (int tm = 0; tm < NT; ++tm) acc[tm] = mfma16(tA[tm][s], bv, acc[tm]);
+        }
+        if (ok) {
+#pragma unroll
+            for (int tm = 0; tm < NT / 2; ++tm)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = tm * 16 + 4 * lq + r;  // complex row in the pivot
+                    Z[(int64_t)rows[row] * LD + c] = make_float2(acc[tm][r], acc[tm + NT / 2][r]);
+                }
         }
 #pragma unroll
-        for (int j = 0; j < B2; ++j) Z[(int64_t)rows[j] * LD + c] = zout[j];
+        for (int s = 0; s < B2 / 4; ++s) zin[s] = znx[s];
     }
 }
 
@@ -559,10 +638,6 @@ hipError_t launch_jacobi_round(hipStream_t s, int b, const mpsx_two_site_op *d, 
                                int64_t pairs, int round, double tol, double tol_inner,
                                int inner_sweeps) {
     switch (b) {
-        case 4:
-            hipLaunchKernelGGL(k_jacobi_round<8>, dim3((unsigned)pairs), dim3(256), 0, s, d, count, ws,
-                               round, tol, tol_inner, inner_sweeps);
-            break;
         case 8:
             hipLaunchKernelGGL(k_jacobi_round<16>, dim3((unsigned)pairs), dim3(256), 0, s, d, count, ws,
                                round, tol, tol_inner, inner_sweeps);
