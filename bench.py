@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--inner", type=int, default=0)
     ap.add_argument("--workload", default="large",
-                    choices=["large", "uneven", "served", "plain16", "qutrit"],
+                    choices=["large", "uneven", "served", "plain16", "qutrit", "copy"],
                     help="large (default, the metric line) | uneven | served | plain16 | qutrit")
     ap.add_argument("--circuits", type=int, default=512, help="uneven: circuits in the batch")
     ap.add_argument("--plain-sample", type=int, default=16,
@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--depth", type=int, default=0, help="qutrit: brickwork depth (0 = 16)")
     return ap.parse_args()
 
 
@@ -61,6 +62,53 @@ def base_line(metric, value, unit, args, ms_per_step, config, dtype="c64"):
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
             "data": "synthetic (seeded Haar gates)", "config": config}
+
+
+def run_copy(args, eng):
+    """Calibration for the HBM counters: k_scale with factor 1 reads and
+    writes exactly 8 bytes per complex element (512 MiB each way)."""
+    import torch
+    x = torch.zeros(64 << 20, dtype=torch.complex64, device=eng.dev)
+    for _ in range(3):
+        eng.scale([x], 1.0)
+    torch.cuda.synchronize()
+    emit({"workload": "copy", "bytes_read_per_launch": x.numel() * 8,
+          "bytes_written_per_launch": x.numel() * 8})
+
+
+def run_qutrit(args, eng, svd):
+    """Precision stress: 48 qutrits, brickwork depth 16 of 9x9 Haar gates,
+    maxsvals=1024 (theta up to 3072 x 3072); one pass, layer-batched."""
+    import torch
+    from mpsim_amd import _native as nat
+    from mpsim_amd.batch import prepare, run_batched
+    from mpsim_amd.circuits import brickwork_haar, circuit_work
+    depth = args.depth or 16
+    circ = brickwork_haar(48, depth, 3, maxsvals=1024, seed=4)
+    flops, _, updates = circuit_work(circ)
+    svd_f, _ = svd_flops(circ, eng.svd.block_switch or 64)
+    dc = prepare(circ, eng)
+    torch.cuda.synchronize()
+    nat.profile_read(reset=True)
+    nat.profile_enable(True)
+    t0 = time.perf_counter()
+    run_batched([dc], eng, svd)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    nat.profile_enable(False)
+    kern = nat.profile_read(reset=True)
+    jr_ms, jr_n = kern.get("k_jacobi_round<32>", (0.0, 0))
+    ach = svd_f["k_jacobi_round<32>"] / (jr_ms * 1e-3) / 1e12 if jr_ms else 0.0
+    line = base_line("2-site gate updates/s, 48 qutrits chi<=1024", updates / dt, "2-site updates/s",
+                     args, 1e3 * dt, {"workload": f"48 qutrits, brickwork depth {depth}, 9x9 Haar, "
+                                      "maxsvals=1024", "updates": updates})
+    line["roofline"] = {"kernel": "k_jacobi_round<32>", "bound": "mfma", "achieved": round(ach, 3),
+                        "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
+                        "frac": round(ach / PEAK_FP32_MFMA, 5), "launches": jr_n}
+    line["extra"] = {"kernel_ms": {k: round(v[0], 2) for k, v in sorted(kern.items())},
+                     "bond_dimensions_max": max(dc.mps.bond_dimensions()),
+                     "algorithmic_tflops_per_s": round(flops / dt / 1e12, 3)}
+    emit(line)
 
 
 def run_uneven(args, eng, svd):
@@ -140,13 +188,16 @@ def run_served(args, eng, svd):
     emit(line)
 
 
-def svd_flops(circuit):
-    """Golub-Reinsch SVD flops (the SVD term of the per-update work) and
-    full per-update flops, summed over the circuit."""
+def svd_flops(circuit, switch=64):
+    """Golub-Reinsch SVD flops (the SVD term of the per-update work), split by
+    the Jacobi kernel that runs the update (32-row pivots when the rows to
+    orthogonalise, d*chi_l, exceed `switch`), and the full per-update flops,
+    summed over the circuit. Returns ({kernel: flops}, full)."""
     from mpsim_amd.circuits import two_site_flops
     n, d = circuit.n, circuit.d
     bonds = [1] * (n + 1)
-    svd = full = 0.0
+    svd = {"k_jacobi_round<16>": 0.0, "k_jacobi_round<32>": 0.0}
+    full = 0.0
     for layer in circuit.layers:
         for q, _ in layer:
             if len(q) != 2:
@@ -157,7 +208,8 @@ def svd_flops(circuit):
             f = two_site_flops(cl, cm, cr, d, k)
             M, N = d * cl, d * cr
             p, qq = max(M, N), min(M, N)
-            svd += 4.0 * (4.0 * p * p * qq + 8.0 * p * qq * qq + 9.0 * qq ** 3)
+            kern = "k_jacobi_round<32>" if M > switch else "k_jacobi_round<16>"
+            svd[kern] += 4.0 * (4.0 * p * p * qq + 8.0 * p * qq * qq + 9.0 * qq ** 3)
             full += f
             bonds[i + 1] = k
     return svd, full
@@ -189,13 +241,17 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     eng = Engine.get("cuda:0")
     svd = SvdConfig(block=args.block, inner_sweeps=args.inner)
+    if args.workload == "copy":
+        return run_copy(args, eng)
     if args.workload == "uneven":
         return run_uneven(args, eng, svd)
     if args.workload == "served":
         return run_served(args, eng, svd)
+    if args.workload == "qutrit":
+        return run_qutrit(args, eng, svd)
     circ = brickwork_haar(64, 20, 2, maxsvals=256, seed=2)
     flops, bytes_, updates = circuit_work(circ)
-    svd_f, _ = svd_flops(circ)
+    svd_f, _ = svd_flops(circ, eng.svd.block_switch or 64)
 
     dc = prepare(circ, eng)
     for _ in range(args.warmup):
@@ -219,26 +275,29 @@ def main():
     value = updates * args.steps / total
     ms_per_step = 1e3 * total / args.steps
 
-    dom = max(kern.items(), key=lambda kv: kv[1][0]) if kern else ("k_jacobi_round", (0.0, 1))
-    jr_ms, jr_n = kern.get("k_jacobi_round", (0.0, 0))
-    achieved = (svd_f * args.steps) / (jr_ms * 1e-3) / 1e12 if jr_ms > 0 else 0.0
+    dom = max(kern.items(), key=lambda kv: kv[1][0])[0] if kern else "k_jacobi_round<32>"
+    jr_ms, jr_n = kern.get(dom, (0.0, 0))
+    dom_f = svd_f.get(dom, 0.0) * args.steps
+    achieved = dom_f / (jr_ms * 1e-3) / 1e12 if jr_ms > 0 else 0.0
     traffic = load_traffic()
     roofline = {
-        "kernel": "k_jacobi_round",
+        "kernel": dom,
         "bound": "mfma",
         "achieved": round(achieved, 3),
         "peak": PEAK_FP32_MFMA,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP32_MFMA, 5),
-        "traffic": traffic.get("bytes_per_launch") if traffic else None,
+        "traffic": (traffic.get("kernels", {}).get(dom, {}).get("bytes_per_launch")
+                    if traffic else None),
         "launches": jr_n,
         "avg_launch_us": round(1e3 * jr_ms / max(jr_n, 1), 3),
-        "algorithmic_flops_per_launch": (svd_f * args.steps) / max(jr_n, 1),
+        "algorithmic_flops_per_launch": dom_f / max(jr_n, 1),
+        "algorithmic_unit": "SVD term 4(4p^2q+8pq^2+9q^3) of the updates this kernel runs",
     }
 
     extra = {
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in sorted(kern.items())},
-        "dominant_kernel": dom[0],
+        "dominant_kernel": dom,
         "sweeps_last_layer": eng.last_sweeps,
         "algorithmic_tflops_per_s": round(flops * args.steps / total / 1e12, 3),
         "algorithmic_hbm_gb_per_s": round(bytes_ * args.steps / total / 1e9, 3),

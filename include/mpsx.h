@@ -90,6 +90,12 @@ int mpsx_apply_one_site(mpsx_stream stream, const mpsx_site_op *h,
  * The caller computes k = min(d*chi_l, d*chi_r, maxsvals) (core.py:1110-1130).
  * The SVD is a blocked one-sided Jacobi on the side that is not scaled by S,
  * so the scaled side never divides by a singular value.
+ *
+ * Plain split: with gate == NULL and d == 1 the op factorises the M x N
+ * matrix stored row-major at `a` (M = chi_l, N = chi_r; `b`, chi_m unused)
+ * the same way. This replaces the other tn.split_node(_full_svd) calls of
+ * the reference: MPS.from_wavefunction (core.py:314-319) and the
+ * orthonormalisation after non-unitary gates (core.py:877-882, 924-929).
  * ---------------------------------------------------------------------- */
 /* Launch plan of one problem. Filled by mpsx_two_site_plan; callers copy it
  * to the device descriptor unchanged and never edit it. */
@@ -129,6 +135,9 @@ typedef struct {
                            device-side early exit)                         */
     int32_t inner_sweeps;/* pivot eigensolver sweep limit; 0 = default     */
     double tol;         /* relative orthogonality tolerance; 0 = default   */
+    int32_t block_switch;/* with block = 0: problems with more than this many
+                           rows use b = 16, the others b = 8; 0 = default 64 */
+    int32_t reserved;
 } mpsx_svd_opts;
 
 /* Validates the batch, fills every h[i].plan and returns the workspace bytes

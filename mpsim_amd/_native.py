@@ -43,7 +43,8 @@ GEMM_CONJ_A, GEMM_CONJ_B, GEMM_ACCUM = 1, 2, 4
 class SvdOpts(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("max_sweeps", ctypes.c_int32),
                 ("check_every", ctypes.c_int32), ("inner_sweeps", ctypes.c_int32),
-                ("tol", ctypes.c_double)]
+                ("tol", ctypes.c_double), ("block_switch", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class MpsxError(RuntimeError):

@@ -13,9 +13,10 @@ Known, documented differences from the reference:
   - The per-gate norm the reference appends to `_norms` after every
     two-qudit gate (core.py:1160-1161, "only for convenience in
     benchmarking") is off by default; MPS(track_norms=True) turns it on.
-  - A non-unitary one-qudit gate is applied and the state renormalised to
-    its previous norm (core.py:814-845), but the bond-shrinking
-    orthonormalisation SVD (core.py:847-939) is not performed.
+  - MPS.from_wavefunction splits with S absorbed right (U, S V^H); the
+    reference's tn.split_node absorbs sqrt(S) into both sides. The state,
+    norms and bond dimensions are the same; only the gauge of the site
+    tensors differs.
   - `maxsvals` < 0 raises ValueError (the reference slices with it).
 """
 from __future__ import annotations
@@ -26,7 +27,7 @@ from typing import List, Optional, Sequence, Tuple, Union
 import numpy as np
 
 from . import _native as nat
-from .engine import Engine, TwoSiteItem
+from .engine import Engine, SplitItem, TwoSiteItem
 from .gates import (Node, as_tensor, cnot, haar_random_unitary, hgate, is_unitary, rgate,
                     swap, xgate)
 
@@ -306,12 +307,112 @@ class MPS:
                              f"but should have MPS qudit dimension = {self._qudit_dimension}")
         unitary = is_unitary(g)
         renorm = kwargs.get("renormalize_after_non_unitary") is not False
+        ortho = kwargs.get("ortho_after_non_unitary") is not False
         norm = self.norm() if (not unitary and renorm) else None
         t = self._tensors[node_index]
         if t.numel():
             self._engine.one_site([(t, t, self._engine.gate(g))])
+        if not unitary and ortho:  # core.py:828-841
+            if node_index == 0:
+                self.orthonormalize_right_edge_of(node_index)
+            elif node_index == self._nqudits - 1:
+                self.orthonormalize_left_edge_of(node_index)
+            else:
+                self.orthonormalize_right_edge_of(node_index)
+                self.orthonormalize_left_edge_of(node_index)
         if norm is not None:
             self.renormalize(norm)
+
+    # ------------------------------------------------------------ splits
+    def _split_keep(self, mat, absorb_left: bool, threshold: float):
+        """SVD of a device matrix keeping the values TensorNetwork keeps for
+        max_truncation_err = threshold * norm(): the number of tails
+        sqrt(sum of the j smallest s^2) that exceed the error (its numpy
+        backend's rule for split_node_full_svd)."""
+        M, N = mat.shape
+        full = min(M, N)
+        res = self._engine.two_site([SplitItem(mat, full, absorb_left, want_s=True)])[0]
+        err = threshold * self.norm()
+        s = res.s.double().cpu().numpy() if full else np.zeros(0)
+        tails = np.sqrt(np.cumsum(np.square(s[::-1])))
+        keep = min(full, int(np.count_nonzero(tails > err)))
+        a = res.a_out.reshape(M, full)[:, :keep].contiguous()
+        b = res.b_out.reshape(full, N)[:keep].contiguous()
+        return a, b, keep
+
+    def orthonormalize_right_edge_of(self, node_index: int, threshold: float = 1e-8) -> None:
+        """core.py:847-892: node = U, right neighbour = S V^H . neighbour."""
+        if not 0 <= node_index < self._nqudits - 1:
+            raise ValueError("Invalid edge index.")
+        torch = self._engine.torch
+        t, nxt = self._tensors[node_index], self._tensors[node_index + 1]
+        cl, d, cr = t.shape
+        u, sv, k = self._split_keep(t.reshape(cl * d, cr), False, threshold)
+        _, d2, cr2 = nxt.shape
+        new = torch.empty((k, d2, cr2), dtype=torch.complex64, device=self._engine.dev)
+        self._engine.gemm([dict(A=sv, B=nxt, C=new, m=k, n=d2 * cr2, k=cr, lda=cr,
+                                ldb=d2 * cr2, ldc=d2 * cr2)])
+        self._tensors[node_index] = u.reshape(cl, d, k)
+        self._tensors[node_index + 1] = new
+
+    def orthonormalize_left_edge_of(self, node_index: int, threshold: float = 1e-8) -> None:
+        """core.py:894-939: node = V^H, left neighbour = neighbour . U S."""
+        if not 0 < node_index <= self._nqudits - 1:
+            raise ValueError("Invalid edge index.")
+        torch = self._engine.torch
+        t, prv = self._tensors[node_index], self._tensors[node_index - 1]
+        cl, d, cr = t.shape
+        us, v, k = self._split_keep(t.reshape(cl, d * cr), True, threshold)
+        cl2, d2, _ = prv.shape
+        new = torch.empty((cl2, d2, k), dtype=torch.complex64, device=self._engine.dev)
+        self._engine.gemm([dict(A=prv, B=us, C=new, m=cl2 * d2, n=k, k=cl, lda=cl, ldb=k, ldc=k)])
+        self._tensors[node_index] = v.reshape(k, d, cr)
+        self._tensors[node_index - 1] = new
+
+    @staticmethod
+    def from_wavefunction(wavefunction, nqudits: int, qudit_dimension: int = 2,
+                          tensor_prefix: str = "q", *, device=None) -> "MPS":
+        """core.py:245-328: successive SVD splits of a state vector (site 0
+        most significant), keeping every singular value."""
+        if not isinstance(wavefunction, (list, tuple, np.ndarray)):
+            raise TypeError("Invalid type for wavefunction.")
+        wf = np.array(wavefunction)
+        if wf.ndim != 1:
+            raise ValueError("Invalid shape for wavefunction. Should be a vector.")
+        if nqudits < 2:
+            raise ValueError("At least two qudits are required.")
+        d = qudit_dimension
+        if wf.size != d ** nqudits:
+            raise ValueError(
+                "Mismatch between wavefunction, qudit_dimension, and nqudits. "
+                f"Expected {d ** nqudits} elements in the wavefunction, but wavefunction has "
+                f"{wf.size} elements.")
+        mps = MPS(nqudits, d, tensor_prefix, device=device, _empty=True)
+        eng = mps._engine
+        rest = eng.torch.from_numpy(wf.astype(np.complex64)).to(eng.dev)
+        chi = 1
+        tensors = []
+        for i in range(nqudits - 1):
+            M = chi * d
+            N = rest.numel() // M
+            k = min(M, N)
+            res = eng.two_site([SplitItem(rest.reshape(M, N), k)])[0]
+            tensors.append(res.a_out.reshape(chi, d, k))
+            rest = res.b_out.reshape(k * N)
+            chi = k
+        tensors.append(rest.reshape(chi, d, 1))
+        mps._tensors = tensors
+        return mps
+
+    def expectation(self, observable: MPSOperation) -> float:
+        """core.py:723-751: <psi|O|psi> for a Hermitian MPSOperation."""
+        if not observable.is_hermitian():
+            raise ValueError("Observable is not Hermitian.")
+        if observable.qudit_dimension != self._qudit_dimension:
+            raise ValueError("Dimension mismatch between observable and MPS.")
+        other = self.copy()
+        other.apply(observable)
+        return float(self.inner_product(other).real)
 
     def apply_one_qudit_gate_to_all(self, gate) -> None:
         for i in range(self._nqudits):

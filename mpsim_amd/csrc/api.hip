@@ -48,16 +48,18 @@ constexpr int64_t kGridLimit = 2147483647LL;
 struct Resolved {
     int block, max_sweeps, check_every, inner_sweeps;
     double tol;
+    int block_switch;
 };
 
 Resolved resolve(const mpsx_svd_opts *o) {
-    Resolved r{0, kDefaultMaxSweeps, 1, kDefaultInnerSweeps, kDefaultTol};
+    Resolved r{0, kDefaultMaxSweeps, 1, kDefaultInnerSweeps, kDefaultTol, 64};
     if (o) {
         if (o->block) r.block = o->block;
         if (o->max_sweeps) r.max_sweeps = o->max_sweeps;
         r.check_every = o->check_every;
         if (o->inner_sweeps) r.inner_sweeps = o->inner_sweeps;
         if (o->tol > 0) r.tol = o->tol;
+        if (o->block_switch > 0) r.block_switch = o->block_switch;
     }
     return r;
 }
@@ -111,9 +113,17 @@ hipError_t timed(const char *name, hipStream_t s, F &&fn) {
     return hipSuccess;
 }
 
+// Workgroups of the theta phase: 32x32 (l, r) tiles of a two-site problem,
+// or one per matrix row of a plain split.
+int64_t theta_tiles(const mpsx_two_site_op &o) {
+    if (o.gate == nullptr) return o.chi_l;
+    return ((int64_t)o.chi_l + THETA_TL - 1) / THETA_TL * (((int64_t)o.chi_r + THETA_TR - 1) / THETA_TR);
+}
+
 // Rows per Jacobi block: 2b-row pivots of 16 for small problems, 32 once
-// there are enough rows to fill several pivots.
-int auto_block(int C) { return C <= 64 ? 8 : 16; }
+// there are more than `switch_rows` rows (the autotuner measures the
+// crossover; 64 by default).
+int auto_block(int C, int switch_rows) { return C <= switch_rows ? 8 : 16; }
 
 int *poll_buffer() {
     thread_local int *buf = nullptr;
@@ -236,7 +246,9 @@ int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts
     int64_t off = kHeaderBytes, tiles = 0, pairs = 0, rows = 0, outs = 0;
     for (int i = 0; i < count; ++i) {
         mpsx_two_site_op &o = h[i];
-        if (o.d < 2 || o.d > 4) return fail(MPSX_EINVAL, "op %d: d=%d not in {2,3,4}", i, o.d);
+        const bool split = o.gate == nullptr;
+        if (split ? o.d != 1 : (o.d < 2 || o.d > 4))
+            return fail(MPSX_EINVAL, "op %d: d=%d not in {2,3,4} (or 1 for a plain split)", i, o.d);
         if (o.chi_l < 0 || o.chi_m < 0 || o.chi_r < 0)
             return fail(MPSX_EINVAL, "op %d: negative bond dimension", i);
         const int64_t M = (int64_t)o.d * o.chi_l, N = (int64_t)o.d * o.chi_r;
@@ -244,15 +256,16 @@ int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts
             return fail(MPSX_EINVAL, "op %d: k=%d outside [0, min(%lld,%lld)]", i, o.k,
                         (long long)M, (long long)N);
         if (M > 65536 || N > 65536) return fail(MPSX_EINVAL, "op %d: matrix too large", i);
-        if (M * N > 0 && o.chi_m > 0 && (!o.a || !o.b)) return fail(MPSX_EINVAL, "op %d: null input", i);
-        if (M * N > 0 && !o.gate) return fail(MPSX_EINVAL, "op %d: null gate", i);
+        if (M * N > 0 && !split && o.chi_m > 0 && (!o.a || !o.b))
+            return fail(MPSX_EINVAL, "op %d: null input", i);
+        if (M * N > 0 && split && !o.a) return fail(MPSX_EINVAL, "op %d: null input", i);
         if (M * o.k > 0 && !o.a_out) return fail(MPSX_EINVAL, "op %d: null a_out", i);
         if (N * o.k > 0 && !o.b_out) return fail(MPSX_EINVAL, "op %d: null b_out", i);
         const bool left_absorb = (o.flags & 1) != 0;
         mpsx_plan &p = o.plan;
         p.C = (int32_t)(left_absorb ? N : M);
         p.R = (int32_t)(left_absorb ? M : N);
-        p.b = r.block ? r.block : auto_block(p.C);
+        p.b = r.block ? r.block : auto_block(p.C, r.block_switch);
         if (p.C == 0) {
             p.nb = 0;
             p.Cp = 0;
@@ -270,7 +283,7 @@ int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts
         p.row_base = rows;
         p.out_base = outs;
         off += problem_bytes(p);
-        tiles += ((int64_t)o.chi_l + THETA_TL - 1) / THETA_TL * (((int64_t)o.chi_r + THETA_TR - 1) / THETA_TR);
+        tiles += theta_tiles(o);
         pairs += p.nb / 2;
         rows += p.Cp;
         outs += M + o.k;
@@ -298,7 +311,7 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
     // consistent (a table that was not planned, or edited, is rejected).
     int64_t need = kHeaderBytes, tiles = 0, pairs = 0, rows = 0, outs = 0;
     int max_nb = 0;
-    bool has_d[5] = {false, false, false, false, false};
+    bool has_d[5] = {false, false, false, false, false};  // index 1: plain splits
     bool has_b[17] = {false};
     for (int i = 0; i < count; ++i) {
         const mpsx_two_site_op &o = h[i];
@@ -306,10 +319,10 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
         if (p.ws_off != need || p.tile_base != tiles || p.pair_base != pairs || p.row_base != rows ||
             p.out_base != outs)
             return fail(MPSX_EINVAL, "op %d: plan missing or stale (call mpsx_two_site_plan)", i);
-        if (o.d < 2 || o.d > 4 || (p.b != 8 && p.b != 16))
+        if (o.d < 1 || o.d > 4 || (p.b != 8 && p.b != 16))
             return fail(MPSX_EINVAL, "op %d: invalid plan", i);
         need += problem_bytes(p);
-        tiles += ((int64_t)o.chi_l + THETA_TL - 1) / THETA_TL * (((int64_t)o.chi_r + THETA_TR - 1) / THETA_TR);
+        tiles += theta_tiles(o);
         pairs += p.nb / 2;
         rows += p.Cp;
         outs += (int64_t)o.d * o.chi_l + o.k;
@@ -325,7 +338,7 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
     const bool can_poll = r.check_every > 0 && cap == hipStreamCaptureStatusNone;
 
     if (rows > 0) HIP_TRY(timed("k_init_rows", stream, [&] { return launch_init_rows(stream, d, count, ws, rows); }), "k_init_rows");
-    for (int D = 2; D <= 4; ++D)
+    for (int D = 1; D <= 4; ++D)
         if (has_d[D] && tiles > 0) HIP_TRY(timed("k_theta", stream, [&] { return launch_theta(stream, D, d, count, ws, tiles); }), "k_theta");
     if (rows > 0) {
         HIP_TRY(timed("k_rownorm", stream, [&] { return launch_rownorm(stream, d, count, ws, rows); }), "k_rownorm");
@@ -340,7 +353,7 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
             for (int round = 0; round < max_nb - 1; ++round)
                 for (int b : {8, 16})
                     if (has_b[b])
-                        HIP_TRY(timed("k_jacobi_round", stream,
+                        HIP_TRY(timed(b == 8 ? "k_jacobi_round<16>" : "k_jacobi_round<32>", stream,
                                       [&] {
                                           return launch_jacobi_round(stream, b, d, count, ws, pairs, round,
                                                                      r.tol, r.tol * 0.01, r.inner_sweeps);

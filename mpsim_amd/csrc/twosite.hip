@@ -99,7 +99,7 @@ k_theta(const mpsx_two_site_op *ops, int count, char *ws) {
     const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
                                OP_FIELD(plan.tile_base), blockIdx.x);
     const mpsx_two_site_op &op = ops[pi];
-    if (op.d != D) return;
+    if (op.d != D || op.gate == nullptr) return;
     const int chi_l = op.chi_l, chi_m = op.chi_m, chi_r = op.chi_r;
     const mpsx_plan p = op.plan;
     const int tiles_r = (chi_r + TR - 1) / TR;
@@ -193,6 +193,26 @@ k_theta(const mpsx_two_site_op *ops, int count, char *ws) {
                 }
         }
     }
+}
+
+// Plain split: one workgroup per row m of the M x N input matrix, copied
+// into X (or X^T for absorb-left problems).
+__global__ void __launch_bounds__(256)
+k_load(const mpsx_two_site_op *ops, int count, char *ws) {
+    const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
+                               OP_FIELD(plan.tile_base), blockIdx.x);
+    const mpsx_two_site_op &op = ops[pi];
+    if (op.gate != nullptr) return;
+    const mpsx_plan p = op.plan;
+    const int m = (int)(blockIdx.x - p.tile_base);
+    if (m >= op.chi_l) return;
+    const int N = op.chi_r, LD = p.LD;
+    const float2 *src = reinterpret_cast<const float2 *>(op.a) + (int64_t)m * N;
+    float2 *Z = z_ptr(ws, p);
+    if (!(op.flags & 1))
+        for (int n = threadIdx.x; n < N; n += 256) Z[(int64_t)m * LD + n] = src[n];
+    else
+        for (int n = threadIdx.x; n < N; n += 256) Z[(int64_t)n * LD + m] = src[n];
 }
 
 // ---------------------------------------------------------------- norms
@@ -616,6 +636,7 @@ hipError_t launch_init_rows(hipStream_t s, const mpsx_two_site_op *d, int count,
 hipError_t launch_theta(hipStream_t s, int D, const mpsx_two_site_op *d, int count, char *ws,
                         int64_t tiles) {
     switch (D) {
+        case 1: hipLaunchKernelGGL(k_load, dim3((unsigned)tiles), dim3(256), 0, s, d, count, ws); break;
         case 2: hipLaunchKernelGGL(k_theta<2>, dim3((unsigned)tiles), dim3(256), 0, s, d, count, ws); break;
         case 3: hipLaunchKernelGGL(k_theta<3>, dim3((unsigned)tiles), dim3(256), 0, s, d, count, ws); break;
         case 4: hipLaunchKernelGGL(k_theta<4>, dim3((unsigned)tiles), dim3(256), 0, s, d, count, ws); break;

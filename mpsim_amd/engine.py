@@ -24,10 +24,11 @@ class SvdConfig:
     check_every: int = 1
     inner_sweeps: int = 0
     tol: float = 0.0
+    block_switch: int = 0   # 0 = library default (64 rows)
 
     def to_c(self) -> nat.SvdOpts:
         return nat.SvdOpts(self.block, self.max_sweeps, self.check_every, self.inner_sweeps,
-                           self.tol)
+                           self.tol, self.block_switch, 0)
 
 
 @dataclass
@@ -35,6 +36,15 @@ class TwoSiteItem:
     a: "object"            # torch [chi_l, d, chi_m]
     b: "object"            # torch [chi_m, d, chi_r]
     gate: "object"         # torch complex64 [d^2, d^2] on device
+    k: int
+    absorb_left: bool = False
+    want_s: bool = False
+
+
+@dataclass
+class SplitItem:
+    """A plain M x N matrix to factorise like a two-site theta."""
+    mat: "object"          # torch complex64 [M, N], contiguous
     k: int
     absorb_left: bool = False
     want_s: bool = False
@@ -61,7 +71,8 @@ class Engine:
         self.dev = nat.device(dev)
         self._ws = None
         self._gates: "OrderedDict[bytes, object]" = OrderedDict()
-        self.svd = SvdConfig()
+        from .tune import tuned_config
+        self.svd = tuned_config()
         self.last_sweeps = 0
 
     @classmethod
@@ -125,16 +136,22 @@ class Engine:
         outs: List[TwoSiteResult] = []
         disc = torch.empty(n, dtype=torch.float64, device=self.dev)
         for i, it in enumerate(items):
-            cl, d, cm = it.a.shape
-            cm2, d2, cr = it.b.shape
-            if cm != cm2 or d != d2:
-                raise ValueError(f"site tensors do not share a bond: {tuple(it.a.shape)} {tuple(it.b.shape)}")
+            if isinstance(it, SplitItem):
+                cl, cr = it.mat.shape
+                d, cm = 1, 0
+                a_ptr, b_ptr, g_ptr = nat.ptr(it.mat), 0, 0
+            else:
+                cl, d, cm = it.a.shape
+                cm2, d2, cr = it.b.shape
+                if cm != cm2 or d != d2:
+                    raise ValueError(f"site tensors do not share a bond: {tuple(it.a.shape)} {tuple(it.b.shape)}")
+                a_ptr, b_ptr, g_ptr = nat.ptr(it.a), nat.ptr(it.b), nat.ptr(it.gate)
             a_out = torch.empty((cl, d, it.k), dtype=torch.complex64, device=self.dev)
             b_out = torch.empty((it.k, d, cr), dtype=torch.complex64, device=self.dev)
             s = torch.empty(it.k, dtype=torch.float32, device=self.dev) if it.want_s else None
-            tab[i]["a"] = nat.ptr(it.a)
-            tab[i]["b"] = nat.ptr(it.b)
-            tab[i]["gate"] = nat.ptr(it.gate)
+            tab[i]["a"] = a_ptr
+            tab[i]["b"] = b_ptr
+            tab[i]["gate"] = g_ptr
             tab[i]["a_out"] = nat.ptr(a_out)
             tab[i]["b_out"] = nat.ptr(b_out)
             tab[i]["s_out"] = nat.ptr(s)
