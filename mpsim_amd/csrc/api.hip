@@ -290,6 +290,8 @@ int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts
     }
     if (tiles > kGridLimit || pairs > kGridLimit || rows > kGridLimit || outs > kGridLimit)
         return fail(MPSX_EINVAL, "batch too large for one launch");
+    // round workgroup -> problem table after the problems
+    off = align_up(off, 256) + align_up(4 * pairs, 256);
     if (bytes) *bytes = (size_t)off;
     g_err.clear();
     return MPSX_OK;
@@ -330,8 +332,11 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
         has_d[o.d] = true;
         if (p.nb > 0) has_b[p.b] = true;
     }
+    const int64_t owner_off = align_up(need, 256);
+    need = owner_off + align_up(4 * pairs, 256);
     if ((int64_t)ws_bytes < need)
         return fail(MPSX_EWORKSPACE, "workspace %zu bytes < %lld needed", ws_bytes, (long long)need);
+    int *owner = reinterpret_cast<int *>(ws + owner_off);
 
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     HIP_TRY(hipStreamIsCapturing(stream, &cap), "hipStreamIsCapturing");
@@ -347,6 +352,8 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
 
     int sweeps = 0;
     bool converged = false;
+    if (pairs > 0)
+        HIP_TRY(timed("k_owner", stream, [&] { return launch_owner(stream, d, count, owner); }), "k_owner");
     if (pairs > 0) {
         int *pin = can_poll ? poll_buffer() : nullptr;
         for (int s = 0; s < r.max_sweeps; ++s) {
@@ -355,7 +362,7 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
                     if (has_b[b])
                         HIP_TRY(timed(b == 8 ? "k_jacobi_round<16>" : "k_jacobi_round<32>", stream,
                                       [&] {
-                                          return launch_jacobi_round(stream, b, d, count, ws, pairs, round,
+                                          return launch_jacobi_round(stream, b, d, owner, ws, pairs, round,
                                                                      r.tol, r.tol * 0.01, r.inner_sweeps);
                                       }),
                                 "k_jacobi_round");

@@ -20,7 +20,8 @@ hipError_t launch_rownorm(hipStream_t s, const mpsx_two_site_op *d, int count, c
                           int64_t rows);
 hipError_t launch_fro(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
                       double null_rel);
-hipError_t launch_jacobi_round(hipStream_t s, int b, const mpsx_two_site_op *d, int count,
+hipError_t launch_owner(hipStream_t s, const mpsx_two_site_op *d, int count, int *owner);
+hipError_t launch_jacobi_round(hipStream_t s, int b, const mpsx_two_site_op *d, const int *owner,
                                char *ws, int64_t pairs, int round, double tol, double tol_inner,
                                int inner_sweeps);
 hipError_t launch_sweep_end(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,

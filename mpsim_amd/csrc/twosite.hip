@@ -272,29 +272,27 @@ MPSX_DEV mfma_f4 mfma16(float a, float b, mfma_f4 c) {
 
 template <int B2>
 __global__ void __launch_bounds__(256)
-k_jacobi_round(const mpsx_two_site_op *ops, int count, char *ws, int round, double tol,
+k_jacobi_round(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
                double tol_inner, int inner_sweeps) {
     constexpr int b = B2 / 2;
     constexpr int NS = 2 * B2;          // stacked real rows
     constexpr int NT = NS / 16;         // 16-row tiles per side
-    constexpr int KC = 64;              // columns per Gram stage
-    constexpr int TPW = NT * NT / 4;    // Gram tiles per wave
-    static_assert(NT * NT % 4 == 0, "Gram tiles must split over 4 waves");
+    constexpr int RG = B2 / 16;         // 16-row groups of complex pivot rows
 
+    // gp: FP32 partial Grams of waves (0+2) and (1+3); Q reuses the space
+    // once H has been formed.
     __shared__ union {
-        float2 xs[B2][KC + 1];          // Gram stage
-        double g[NS][NS + 1];           // stacked Gram
+        float gp[2][NS][NS];
+        double2 q[B2][B2 + 1];
     } sm;
     __shared__ double2 H[B2][B2 + 1];
-    __shared__ double2 Q[B2][B2 + 1];
     __shared__ int rows[B2];
     __shared__ double prm_c[b];
     __shared__ double2 prm_s[b];
-    __shared__ int prm_p[b], prm_q[b];
     __shared__ int s_any;
+    auto &Q = sm.q;
 
-    const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
-                               OP_FIELD(plan.pair_base), blockIdx.x);
+    const int pi = owner[blockIdx.x];
     const mpsx_plan p = ops[pi].plan;
     if (p.b != b) return;
     if (round >= p.nb - 1) return;
@@ -306,81 +304,89 @@ k_jacobi_round(const mpsx_two_site_op *ops, int count, char *ws, int round, doub
     rr_pair(pair, round, p.nb, &I, &J);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
-    if (tid < B2) rows[tid] = tid < b ? I * b + tid : J * b + (tid - b);
+    auto row_of = [&](int i) { return i < b ? I * b + i : J * b + (i - b); };
+    if (tid < B2) rows[tid] = row_of(tid);
     float2 *Z = z_ptr(ws, p);
     const int LD = p.LD, R = p.R;
-    __syncthreads();
 
-    // ---- Gram G = S S^T on the matrix cores; FP64 across KC-column stages
-    double gacc[TPW][4];
+    // ---- Gram G = S S^T on the matrix cores, S = [Re Xp; Im Xp]. Wave w
+    // takes the 16-column blocks w, w+4, ... of X; lane (l16, lq) loads four
+    // consecutive columns of each of its complex rows g*16 + l16 and feeds
+    // column c0 + 4 lq + s at k-step s (the sum over columns is
+    // order-free; A and B operands use the same order). Operands go from
+    // global memory straight to registers: no staging, no barriers.
+    mfma_f4 acc[NT][NT];
 #pragma unroll
-    for (int t = 0; t < TPW; ++t)
+    for (int u = 0; u < NT; ++u)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) gacc[t][r] = 0.0;
-    // Stage loads are issued one stage ahead into registers (EPT per thread)
-    // so that global latency overlaps the matrix-core work of the stage.
-    constexpr int EPT = B2 * KC / 256;
-    float2 pre[EPT];
+        for (int v = 0; v < NT; ++v) acc[u][v] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+    int64_t goff[RG];
 #pragma unroll
-    for (int u = 0; u < EPT; ++u) {
-        const int e = tid + 256 * u, rr = e / KC, cc = e % KC;
-        pre[u] = cc < R ? Z[(int64_t)rows[rr] * LD + cc] : make_float2(0.f, 0.f);
-    }
-    for (int c0 = 0; c0 < R; c0 += KC) {
+    for (int g = 0; g < RG; ++g) goff[g] = (int64_t)row_of(g * 16 + l16) * LD;
+    for (int c0 = wave * 16; c0 < R; c0 += 64) {
+        float2 v[RG][4];
 #pragma unroll
-        for (int u = 0; u < EPT; ++u) {
-            const int e = tid + 256 * u;
-            sm.xs[e / KC][e % KC] = pre[u];
-        }
-        __syncthreads();
-        if (c0 + KC < R) {
+        for (int g = 0; g < RG; ++g)
 #pragma unroll
-            for (int u = 0; u < EPT; ++u) {
-                const int e = tid + 256 * u, rr = e / KC, cc = e % KC;
-                const int c = c0 + KC + cc;
-                pre[u] = c < R ? Z[(int64_t)rows[rr] * LD + c] : make_float2(0.f, 0.f);
+            for (int k = 0; k < 4; ++k) {
+                const int c = c0 + 4 * lq + k;
+                v[g][k] = c < R ? Z[goff[g] + c] : make_float2(0.f, 0.f);
             }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float op[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) op[t] = t < NT / 2 ? v[t][k].x : v[t - NT / 2][k].y;
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+#pragma unroll
+                for (int w = 0; w < NT; ++w) acc[u][w] = mfma16(op[u], op[w], acc[u][w]);
         }
-        mfma_f4 acc[TPW];
-#pragma unroll
-        for (int t = 0; t < TPW; ++t) acc[t] = mfma_f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-        for (int ks = 0; ks < KC / 4; ++ks) {
-            const int kk = 4 * ks + lq;
-#pragma unroll
-            for (int t = 0; t < TPW; ++t) {
-                const int tile = wave * TPW + t;
-                const int tm = tile / NT, tn = tile % NT;
-                const int ra = tm * 16 + l16, rb = tn * 16 + l16;
-                const float2 va = sm.xs[ra % B2][kk], vb = sm.xs[rb % B2][kk];
-                const float a = ra < B2 ? va.x : va.y;
-                const float bb = rb < B2 ? vb.x : vb.y;
-                acc[t] = mfma16(a, bb, acc[t]);
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < TPW; ++t)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) gacc[t][r] += (double)acc[t][r];
-        __syncthreads();
     }
+    // deterministic reduction of the four wave partials: (w2 + w0), (w3 + w1)
+    // in FP32, then the two halves in FP64 while forming H.
+    if (wave >= 2) {
 #pragma unroll
-    for (int t = 0; t < TPW; ++t) {
-        const int tile = wave * TPW + t;
-        const int tm = tile / NT, tn = tile % NT;
+        for (int u = 0; u < NT; ++u)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) sm.g[tm * 16 + 4 * lq + r][tn * 16 + l16] = gacc[t][r];
+            for (int w = 0; w < NT; ++w)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sm.gp[wave - 2][u * 16 + 4 * lq + r][w * 16 + l16] = acc[u][w][r];
     }
     __syncthreads();
-    for (int e = tid; e < B2 * B2; e += 256) {
-        const int i = e / B2, j = e % B2;
-        const int u = i <= j ? i : j, v = i <= j ? j : i;  // upper triangle, mirrored
-        double re = sm.g[u][v] + sm.g[B2 + u][B2 + v];
-        double im = sm.g[B2 + u][v] - sm.g[u][B2 + v];
-        if (i > j) im = -im;
-        if (i == j) im = 0.0;
-        H[i][j] = make_double2(re, im);
-        Q[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    if (wave < 2) {
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+#pragma unroll
+            for (int w = 0; w < NT; ++w)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sm.gp[wave][u * 16 + 4 * lq + r][w * 16 + l16] += acc[u][w][r];
+    }
+    __syncthreads();
+    double2 hv[(B2 * B2 + 255) / 256];
+#pragma unroll
+    for (int t = 0; t < (B2 * B2 + 255) / 256; ++t) {
+        const int e = tid + 256 * t;
+        if (e < B2 * B2) {
+            const int i = e / B2, j = e % B2;
+            const int u = i <= j ? i : j, v = i <= j ? j : i;  // upper triangle, mirrored
+            auto G = [&](int r, int c) { return (double)sm.gp[0][r][c] + (double)sm.gp[1][r][c]; };
+            const double re = G(u, v) + G(B2 + u, B2 + v);
+            double im = G(B2 + u, v) - G(u, B2 + v);
+            if (i > j) im = -im;
+            if (i == j) im = 0.0;
+            hv[t] = make_double2(re, im);
+        }
+    }
+    __syncthreads();  // gp is dead from here on; Q reuses it
+#pragma unroll
+    for (int t = 0; t < (B2 * B2 + 255) / 256; ++t) {
+        const int e = tid + 256 * t;
+        if (e < B2 * B2) {
+            const int i = e / B2, j = e % B2;
+            H[i][j] = hv[t];
+            Q[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        }
     }
     __syncthreads();
 
@@ -399,7 +405,27 @@ k_jacobi_round(const mpsx_two_site_op *ops, int count, char *ws, int round, doub
     if (!__syncthreads_or(rot)) return;
     if (tid == 0) atomicAdd(&ctl->counter, 1);
 
-    // ---- pivot eigensolve: cyclic Jacobi on H (FP64), Q accumulates
+    // ---- pivot eigensolve: cyclic Jacobi on H (FP64), Q accumulates.
+    // Work items of a step: the b(b+1)/2 upper 2x2 blocks of H and the B2*b
+    // (row, pair) entries of Q; a thread's items are fixed across steps and
+    // the index pairs of a step are computed arithmetically.
+    constexpr int NH = b * (b + 1) / 2, NQ = B2 * b, NIT = (NH + NQ + 255) / 256;
+    int ik1[NIT], ik2[NIT];
+#pragma unroll
+    for (int t = 0; t < NIT; ++t) {
+        const int it = tid + 256 * t;
+        ik1[t] = -1;
+        ik2[t] = -1;
+        if (it < NH) {
+            int k1 = 0, rem = it;
+            while (rem >= b - k1) { rem -= b - k1; ++k1; }
+            ik1[t] = k1;
+            ik2[t] = k1 + rem;
+        } else if (it < NH + NQ) {
+            ik1[t] = -2 - (it - NH) / b;  // Q row, encoded
+            ik2[t] = (it - NH) % b;        // pair
+        }
+    }
     for (int sw = 0; sw < inner_sweeps; ++sw) {
         __syncthreads();  // every wave has read s_any of the previous sweep
         if (tid == 0) s_any = 0;
@@ -411,34 +437,34 @@ k_jacobi_round(const mpsx_two_site_op *ops, int count, char *ws, int round, doub
                 rr_pair(tid, step, B2, &pp, &qq);
                 const double a = H[pp][pp].x, c = H[qq][qq].x;
                 const double2 h = H[pp][qq];
-                const double g = sqrt(h.x * h.x + h.y * h.y);
+                const double g2 = h.x * h.x + h.y * h.y;
                 double cs = 1.0;
                 double2 s = make_double2(0.0, 0.0);
-                if (a > null2 && c > null2 && g > tol_inner * sqrt(a * c) && g > 0.0) {
-                    const double rg = 1.0 / g;
-                    const double zeta = 0.5 * (c - a) * rg;
-                    const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                    cs = 1.0 / sqrt(1.0 + t * t);
-                    const double sn = t * cs * rg;
-                    s = make_double2(sn * h.x, sn * h.y);  // sn * e^{i phi}
+                if (a > null2 && c > null2 && g2 > tol_inner * tol_inner * a * c && g2 > 0.0) {
+                    // tan of the rotation angle t = w |h| with
+                    // w = sign(tau) / (|tau| + sqrt(tau^2 + |h|^2)), tau = (c - a)/2;
+                    // s = sn e^{i phi} = (t cs / |h|) h = w cs h.
+                    const double tau = 0.5 * (c - a);
+                    const double w = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(tau * tau + g2));
+                    cs = rsqrt(1.0 + w * w * g2);
+                    const double wc = w * cs;
+                    s = make_double2(wc * h.x, wc * h.y);
                     s_any = 1;
                     step_rot = 1;  // (cs can round to exactly 1 while s != 0)
                 }
-                prm_p[tid] = pp;
-                prm_q[tid] = qq;
                 prm_c[tid] = cs;
                 prm_s[tid] = s;
             }
             if (!__syncthreads_or(step_rot)) continue;  // every pair already diagonal
             // H <- J^H H J on 2x2 blocks (upper blocks, mirrored), Q <- Q J.
             // J = [[c, s], [-conj(s), c]] for each index pair (p, q).
-            constexpr int NH = b * (b + 1) / 2, NQ = B2 * b;
-            for (int it = tid; it < NH + NQ; it += 256) {
-                if (it < NH) {
-                    int k1 = 0, rem = it;
-                    while (rem >= b - k1) { rem -= b - k1; ++k1; }
-                    const int k2 = k1 + rem;
-                    const int p1 = prm_p[k1], q1 = prm_q[k1], p2 = prm_p[k2], q2 = prm_q[k2];
+#pragma unroll
+            for (int t = 0; t < NIT; ++t) {
+                const int k1 = ik1[t], k2 = ik2[t];
+                if (k1 >= 0) {
+                    int p1, q1, p2, q2;
+                    rr_pair(k1, step, B2, &p1, &q1);
+                    rr_pair(k2, step, B2, &p2, &q2);
                     const double c1 = prm_c[k1], c2 = prm_c[k2];
                     const double2 s1 = prm_s[k1], s2 = prm_s[k2];
                     const double2 h00 = H[p1][p2], h01 = H[p1][q2], h10 = H[q1][p2], h11 = H[q1][q2];
@@ -463,10 +489,10 @@ k_jacobi_round(const mpsx_two_site_op *ops, int count, char *ws, int round, doub
                         H[p2][q1] = zconj(n10);
                         H[q2][q1] = zconj(n11);
                     }
-                } else {
-                    const int q = it - NH;
-                    const int row = q / b, k = q % b;
-                    const int pp = prm_p[k], qq = prm_q[k];
+                } else if (k1 <= -2) {
+                    const int row = -2 - k1, k = k2;
+                    int pp, qq;
+                    rr_pair(k, step, B2, &pp, &qq);
                     const double c = prm_c[k];
                     const double2 s = prm_s[k];
                     const double2 qp = Q[row][pp], qv = Q[row][qq];
@@ -538,6 +564,15 @@ k_jacobi_round(const mpsx_two_site_op *ops, int count, char *ws, int round, doub
 #pragma unroll
         for (int s = 0; s < B2 / 4; ++s) zin[s] = znx[s];
     }
+}
+
+// Round workgroup -> problem table (one workgroup per problem), so that the
+// round kernels need no search.
+__global__ void k_owner(const mpsx_two_site_op *ops, int count, int *owner) {
+    const int pi = blockIdx.x;
+    if (pi >= count) return;
+    const mpsx_plan p = ops[pi].plan;
+    for (int j = threadIdx.x; j < p.nb / 2; j += blockDim.x) owner[p.pair_base + j] = pi;
 }
 
 // After each sweep: a problem whose sweep rotated nothing is converged.
@@ -655,16 +690,21 @@ hipError_t launch_fro(hipStream_t s, const mpsx_two_site_op *d, int count, char 
     return hipGetLastError();
 }
 
-hipError_t launch_jacobi_round(hipStream_t s, int b, const mpsx_two_site_op *d, int count, char *ws,
-                               int64_t pairs, int round, double tol, double tol_inner,
+hipError_t launch_owner(hipStream_t s, const mpsx_two_site_op *d, int count, int *owner) {
+    hipLaunchKernelGGL(k_owner, dim3((unsigned)count), dim3(256), 0, s, d, count, owner);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_round(hipStream_t s, int b, const mpsx_two_site_op *d, const int *owner,
+                               char *ws, int64_t pairs, int round, double tol, double tol_inner,
                                int inner_sweeps) {
     switch (b) {
         case 8:
-            hipLaunchKernelGGL(k_jacobi_round<16>, dim3((unsigned)pairs), dim3(256), 0, s, d, count, ws,
+            hipLaunchKernelGGL(k_jacobi_round<16>, dim3((unsigned)pairs), dim3(256), 0, s, d, owner, ws,
                                round, tol, tol_inner, inner_sweeps);
             break;
         case 16:
-            hipLaunchKernelGGL(k_jacobi_round<32>, dim3((unsigned)pairs), dim3(256), 0, s, d, count, ws,
+            hipLaunchKernelGGL(k_jacobi_round<32>, dim3((unsigned)pairs), dim3(256), 0, s, d, owner, ws,
                                round, tol, tol_inner, inner_sweeps);
             break;
         default: return hipErrorInvalidValue;

@@ -68,7 +68,8 @@ def tune(engine=None, chis=(8, 16, 32, 64, 128, 256), reps: int = 2,
     eng = engine or Engine.get()
     rows: List[Dict] = []
     for chi in chis:
-        count = max(1, min(64, 16384 // (chi * chi) * 8))
+        # enough problems for ~1024 pivot workgroups per round (a full device)
+        count = int(max(1, min(64, -(-1024 // max(1, (2 * chi) // 16)))))
         items = _problems(eng, chi, count, seed=chi)
         best = None
         for block in (8, 16):
