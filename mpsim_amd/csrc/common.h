@@ -52,6 +52,25 @@ MPSX_HD void rr_pair(int p, int r, int n, int *lo, int *hi) {
     *hi = a < b ? b : a;
 }
 
+// Ordering of one inner sweep over a 2b-row pivot (rows 0..b-1 from block I,
+// b..2b-1 from block J): steps 0..b-1 pair I row k with J row (k + step) mod b
+// (the cross pairs); steps b..2b-2 pair the rows inside I and inside J by the
+// circle method. Every pair of the pivot meets once per sweep; intra-block
+// pairs, already orthogonal once the iteration settles, come last so their
+// steps find nothing to rotate.
+MPSX_HD void inner_pair(int k, int step, int b, int *p, int *q) {
+    if (step < b) {
+        *p = k;
+        *q = b + (k + step) % b;
+    } else if (k < b / 2) {
+        rr_pair(k, step - b, b, p, q);
+    } else {
+        rr_pair(k - b / 2, step - b, b, p, q);
+        *p += b;
+        *q += b;
+    }
+}
+
 // Binary search: index i with base[i] <= x < base[i+1] for a table of
 // `count` records of byte size `stride` whose int64 field at `field_off`
 // holds an exclusive prefix sum (records are sorted by construction).

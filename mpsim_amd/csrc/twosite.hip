@@ -325,13 +325,26 @@ k_jacobi_round(const mpsx_two_site_op *ops, const int *owner, char *ws, int roun
     for (int g = 0; g < RG; ++g) goff[g] = (int64_t)row_of(g * 16 + l16) * LD;
     for (int c0 = wave * 16; c0 < R; c0 += 64) {
         float2 v[RG][4];
+        const int cq = c0 + 4 * lq;
+        if (cq + 4 <= R) {  // 32 contiguous bytes per row: two 16-byte loads
 #pragma unroll
-        for (int g = 0; g < RG; ++g)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int c = c0 + 4 * lq + k;
-                v[g][k] = c < R ? Z[goff[g] + c] : make_float2(0.f, 0.f);
+            for (int g = 0; g < RG; ++g) {
+                const float4 *src = reinterpret_cast<const float4 *>(Z + goff[g] + cq);
+                const float4 lo = src[0], hi = src[1];
+                v[g][0] = make_float2(lo.x, lo.y);
+                v[g][1] = make_float2(lo.z, lo.w);
+                v[g][2] = make_float2(hi.x, hi.y);
+                v[g][3] = make_float2(hi.z, hi.w);
             }
+        } else {
+#pragma unroll
+            for (int g = 0; g < RG; ++g)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int c = cq + k;
+                    v[g][k] = c < R ? Z[goff[g] + c] : make_float2(0.f, 0.f);
+                }
+        }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             float op[NT];
@@ -434,7 +447,7 @@ k_jacobi_round(const mpsx_two_site_op *ops, const int *owner, char *ws, int roun
             int step_rot = 0;
             if (tid < b) {
                 int pp, qq;
-                rr_pair(tid, step, B2, &pp, &qq);
+                inner_pair(tid, step, b, &pp, &qq);
                 const double a = H[pp][pp].x, c = H[qq][qq].x;
                 const double2 h = H[pp][qq];
                 const double g2 = h.x * h.x + h.y * h.y;
@@ -463,8 +476,8 @@ k_jacobi_round(const mpsx_two_site_op *ops, const int *owner, char *ws, int roun
                 const int k1 = ik1[t], k2 = ik2[t];
                 if (k1 >= 0) {
                     int p1, q1, p2, q2;
-                    rr_pair(k1, step, B2, &p1, &q1);
-                    rr_pair(k2, step, B2, &p2, &q2);
+                    inner_pair(k1, step, b, &p1, &q1);
+                    inner_pair(k2, step, b, &p2, &q2);
                     const double c1 = prm_c[k1], c2 = prm_c[k2];
                     const double2 s1 = prm_s[k1], s2 = prm_s[k2];
                     const double2 h00 = H[p1][p2], h01 = H[p1][q2], h10 = H[q1][p2], h11 = H[q1][q2];
@@ -492,7 +505,7 @@ k_jacobi_round(const mpsx_two_site_op *ops, const int *owner, char *ws, int roun
                 } else if (k1 <= -2) {
                     const int row = -2 - k1, k = k2;
                     int pp, qq;
-                    rr_pair(k, step, B2, &pp, &qq);
+                    inner_pair(k, step, b, &pp, &qq);
                     const double c = prm_c[k];
                     const double2 s = prm_s[k];
                     const double2 qp = Q[row][pp], qv = Q[row][qq];
