@@ -138,6 +138,10 @@ typedef struct {
     int32_t block_switch;/* with block = 0: problems with more than this many
                            rows use b = 16, the others b = 8; 0 = default 64 */
     int32_t reserved;
+    int32_t *unconverged;/* device int or NULL: incremented once per problem
+                           whose last launched sweep still rotated (the
+                           device-side status when check_every = 0); the
+                           caller zeroes it                                 */
 } mpsx_svd_opts;
 
 /* Validates the batch, fills every h[i].plan and returns the workspace bytes
@@ -201,6 +205,26 @@ int mpsx_profile_enable(int on);
 /* Writes up to `cap` entries; returns the number of distinct kernels. */
 int mpsx_profile_read(char *names, int name_len, double *total_ms, int64_t *launches,
                       int cap, int reset);
+
+/* ------------------------------------------------------------------------
+ * Graph capture of library calls. Every call above launches kernels only
+ * (no allocation, no host synchronisation) when its stream is being
+ * captured, with check_every = 0 for mpsx_two_site, so a fixed schedule of
+ * calls on preallocated buffers can be recorded once and replayed as one
+ * hipGraph. The descriptor tables and buffers the calls name are read at
+ * replay time and must stay alive and unchanged (their contents may be
+ * updated between replays). No counterpart in the reference, which applies
+ * gates one by one from Python (core.py:1221-1247).
+ * ---------------------------------------------------------------------- */
+typedef struct mpsx_graph_s *mpsx_graph;
+
+/* Starts capturing `stream` (not the null stream) in thread-local mode. */
+int mpsx_graph_begin(mpsx_stream stream);
+/* Ends the capture and instantiates it into *out. */
+int mpsx_graph_end(mpsx_stream stream, mpsx_graph *out);
+/* Replays the graph on `stream`. */
+int mpsx_graph_launch(mpsx_graph graph, mpsx_stream stream);
+int mpsx_graph_destroy(mpsx_graph graph);
 
 #ifdef __cplusplus
 }

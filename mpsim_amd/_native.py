@@ -44,7 +44,7 @@ class SvdOpts(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("max_sweeps", ctypes.c_int32),
                 ("check_every", ctypes.c_int32), ("inner_sweeps", ctypes.c_int32),
                 ("tol", ctypes.c_double), ("block_switch", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("reserved", ctypes.c_int32), ("unconverged", ctypes.c_void_p)]
 
 
 class MpsxError(RuntimeError):
@@ -61,7 +61,8 @@ class NotConvergedWarning(RuntimeWarning):
 
 EXPORTS = ("mpsx_version", "mpsx_last_error", "mpsx_device_info", "mpsx_sizeof",
            "mpsx_apply_one_site", "mpsx_two_site_plan", "mpsx_two_site", "mpsx_gemm",
-           "mpsx_scale", "mpsx_profile_enable", "mpsx_profile_read")
+           "mpsx_scale", "mpsx_profile_enable", "mpsx_profile_read", "mpsx_graph_begin",
+           "mpsx_graph_end", "mpsx_graph_launch", "mpsx_graph_destroy")
 
 _lock = threading.Lock()
 _lib: Optional[ctypes.CDLL] = None
@@ -92,9 +93,14 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
         lib.mpsx_profile_enable.argtypes = [i32]
         lib.mpsx_profile_read.argtypes = [ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_int64), i32, i32]
+        lib.mpsx_graph_begin.argtypes = [vp]
+        lib.mpsx_graph_end.argtypes = [vp, ctypes.POINTER(vp)]
+        lib.mpsx_graph_launch.argtypes = [vp, vp]
+        lib.mpsx_graph_destroy.argtypes = [vp]
         for name in ("mpsx_apply_one_site", "mpsx_two_site_plan", "mpsx_two_site", "mpsx_gemm",
                      "mpsx_scale", "mpsx_device_info", "mpsx_profile_enable",
-                     "mpsx_profile_read"):
+                     "mpsx_profile_read", "mpsx_graph_begin", "mpsx_graph_end",
+                     "mpsx_graph_launch", "mpsx_graph_destroy"):
             getattr(lib, name).restype = i32
         sizes = [lib.mpsx_sizeof(i) for i in range(6)]
         want = [SITE_OP.itemsize, TWO_SITE_OP.itemsize, GEMM_OP.itemsize, SCALE_OP.itemsize,

@@ -28,12 +28,50 @@ import numpy as np
 
 from . import _native as nat
 from .engine import Engine, SplitItem, TwoSiteItem
-from .gates import (Node, as_tensor, cnot, haar_random_unitary, hgate, is_unitary, rgate,
+from .gates import (Edge, Node, as_tensor, cnot, haar_random_unitary, hgate, is_unitary, rgate,
                     swap, xgate)
 
 
 class CannotConvertToMPSOperation(Exception):
     pass
+
+
+class BondEdge:
+    """The bond between sites j and j+1 of an MPS. Its two end nodes are
+    named as the reference wires the bonds of a new MPS (core.py:221-229):
+    (q_j, q_{j+1}), except the last bond of a chain of three or more sites,
+    which is wired from the last site, (q_{n-1}, q_{n-2}). Two BondEdges are
+    equal when they are the same bond of the same MPS."""
+
+    def __init__(self, mps: "MPS", j: int) -> None:
+        self._mps = mps
+        self.index = j
+        n = mps.nqudits
+        self._ends = (j + 1, j) if (n >= 3 and j == n - 2) else (j, j + 1)
+
+    @property
+    def node1(self) -> Node:
+        return self._mps.get_node(self._ends[0])
+
+    @property
+    def node2(self) -> Node:
+        return self._mps.get_node(self._ends[1])
+
+    @property
+    def dimension(self) -> int:
+        return int(self._mps.tensors[self.index].shape[2])
+
+    def is_dangling(self) -> bool:
+        return False
+
+    def __eq__(self, other) -> bool:
+        return (isinstance(other, BondEdge) and other._mps is self._mps
+                and other.index == self.index)
+
+    __hash__ = None
+
+    def __repr__(self) -> str:
+        return f"BondEdge({self.index}, dim={self.dimension})"
 
 
 class MPSOperation:
@@ -214,6 +252,21 @@ class MPS:
         i = node_index % self._nqudits
         return Node(t.detach().cpu().numpy(), name=self._prefix + str(i))
 
+    def get_free_edge_of(self, node_index: int, copy: bool = True) -> Edge:
+        """core.py:443-451: the physical leg of a site (axis 1 of the
+        [chi_l, d, chi_r] host copy)."""
+        return self.get_node(node_index, copy).get_edge(1)
+
+    def get_left_connected_edge_of(self, node_index: int) -> Optional["BondEdge"]:
+        """core.py:453-466: the bond left of a site, None for site 0."""
+        i = node_index % self._nqudits
+        return None if i == 0 else BondEdge(self, i - 1)
+
+    def get_right_connected_edge_of(self, node_index: int) -> Optional["BondEdge"]:
+        """core.py:468-481: the bond right of a site, None for the last site."""
+        i = node_index % self._nqudits
+        return None if i == self._nqudits - 1 else BondEdge(self, i)
+
     # ------------------------------------------------------------ read-out
     def wavefunction(self) -> np.ndarray:
         """core.py:483-500: psi with site 0 the most significant digit."""
@@ -287,6 +340,159 @@ class MPS:
             raise ValueError("Norm of MPS is numerically zero, cannot renormalize.")
         f = (to_norm / norm) ** (1.0 / self._nqudits)
         self._engine.scale([t for t in self._tensors if t.numel()], f)
+
+    # ------------------------------------------------------------ reduced states
+    def reduced_density_matrix(self, node_indices: Union[int, Sequence[int]]) -> np.ndarray:
+        """core.py:596-652: the density matrix of the kept sites with every
+        other site traced out, shape (d^m, d^m). Rows are the kept ket legs
+        and columns the kept bra legs, each a big-endian number over the
+        sites in the order `node_indices` lists them (core.py:648-651)."""
+        try:
+            idx = tuple(iter(node_indices))
+        except TypeError:
+            idx = (node_indices,)
+        idx = tuple(int(i) for i in idx)
+        if len(set(idx)) < len(idx):
+            raise ValueError("Node indices contains duplicates.")
+        if min(idx) < 0 or max(idx) > self._nqudits - 1:
+            raise IndexError("One or more invalid node indices.")
+        kept = sorted(idx)
+        rho = self._rdm_device(kept).cpu().numpy()
+        m, d = len(kept), self._qudit_dimension
+        pos = [kept.index(i) for i in idx]
+        rho = rho.reshape([d] * (2 * m)).transpose(pos + [m + p for p in pos])
+        return np.ascontiguousarray(rho).reshape(d ** m, d ** m)
+
+    def _rdm_device(self, kept: Sequence[int]):
+        """rho[a, b] over the kept sites (sorted), a the ket and b the bra
+        multi-index, by one left-to-right sweep of grouped GEMMs.
+
+        The environment after a prefix of sites is E[b, y, a, x]: b / a the
+        bra / ket values of the kept sites so far, y / x the open bra / ket
+        bond. Per site A[x, s, x']:
+          F[b, y, a, (s, x')] = sum_x E[b, y, a, x] A[x, s, x']       (one GEMM)
+          kept:   E'[(b, t), y', (a, s), x'] = sum_y conj(A[y, t, y']) F[b, y, a, s, x']
+                  (one GEMM per b)
+          traced: E'[b, y', a, x'] = sum_{y, s} conj(A[y, s, y']) F[b, y, a, s, x']
+                  (one GEMM per (b, a), accumulated over s)
+        """
+        torch = self._engine.torch
+        dev, eng, d = self._engine.dev, self._engine, self._qudit_dimension
+        keep = set(kept)
+        env = torch.ones((1, 1, 1, 1), dtype=torch.complex64, device=dev)
+        kb = ka = 1
+        for i, a in enumerate(self._tensors):
+            xl, _, xr = a.shape
+            f = torch.empty((kb, xl, ka, d * xr), dtype=torch.complex64, device=dev)
+            eng.gemm([dict(A=env, B=a, C=f, m=kb * xl * ka, n=d * xr, k=xl, lda=xl,
+                           ldb=d * xr, ldc=d * xr)])
+            row_f = ka * d * xr
+            if i in keep:
+                nxt = torch.empty((kb, d, xr, ka, d, xr), dtype=torch.complex64, device=dev)
+                eng.gemm([dict(A=a, B=(f, b * xl * row_f), C=(nxt, b * d * xr * ka * d * xr),
+                               m=d * xr, n=row_f, k=xl, lda=d * xr, ldb=row_f, ldc=row_f,
+                               flags=nat.GEMM_CONJ_A) for b in range(kb)])
+                kb, ka = kb * d, ka * d
+            else:
+                nxt = torch.empty((kb, xr, ka, xr), dtype=torch.complex64, device=dev)
+                for s in range(d):
+                    eng.gemm([dict(A=(a, s * xr), B=(f, b * xl * row_f + c * d * xr + s * xr),
+                                   C=(nxt, b * xr * ka * xr + c * xr), m=xr, n=xr, k=xl,
+                                   lda=d * xr, ldb=row_f, ldc=ka * xr,
+                                   flags=nat.GEMM_CONJ_A | (nat.GEMM_ACCUM if s else 0))
+                              for b in range(kb) for c in range(ka)])
+            env = nxt
+        return env.reshape(kb, ka).transpose(0, 1)
+
+    def _marginals_device(self):
+        """p[i, s] = <psi| (|s><s| on site i) |psi> for every site at once:
+        right environments by a right-to-left sweep, then one left-to-right
+        sweep that splits each site's transfer step by physical value.
+
+        With R~_i[y, x] the environment of the bonds right of site i-1 (bra
+        y, ket x), Q_i = conj(R~_i) is what is stored:
+          G[(x, s), y'] = sum_x' A[(x, s), x'] R~_{i+1}[y', x']  = A Q_{i+1}^H
+          Q_i = A_(y, (s y')) G_(x, (s y'))^H
+        Left sweep with L[y, x]:
+          F = L A;  D_s[y', x'] = sum_y conj(A[y, s, y']) F[y, s, x'];
+          p[i, s] = sum D_s[y', x'] R~_{i+1}[y', x'];  L' = sum_s D_s."""
+        torch = self._engine.torch
+        dev, eng, d, n = self._engine.dev, self._engine, self._qudit_dimension, self._nqudits
+        c64 = torch.complex64
+        q = [None] * (n + 1)
+        q[n] = torch.ones((1, 1), dtype=c64, device=dev)
+        for i in range(n - 1, 0, -1):
+            a = self._tensors[i]
+            xl, _, xr = a.shape
+            g = torch.empty((xl * d, xr), dtype=c64, device=dev)
+            eng.gemm([dict(A=a, B=q[i + 1], C=g, m=xl * d, n=xr, k=xr, lda=xr, ldb=xr, ldc=xr,
+                           flags=nat.GEMM_CONJ_B)])
+            q[i] = torch.empty((xl, xl), dtype=c64, device=dev)
+            eng.gemm([dict(A=a, B=g, C=q[i], m=xl, n=xl, k=d * xr, lda=d * xr, ldb=d * xr,
+                           ldc=xl, flags=nat.GEMM_CONJ_B)])
+        env = torch.ones((1, 1), dtype=c64, device=dev)
+        dsv = []
+        for i, a in enumerate(self._tensors):
+            xl, _, xr = a.shape
+            f = torch.empty((xl, d * xr), dtype=c64, device=dev)
+            eng.gemm([dict(A=env, B=a, C=f, m=xl, n=d * xr, k=xl, lda=xl, ldb=d * xr,
+                           ldc=d * xr)])
+            ds = torch.empty((d, xr, xr), dtype=c64, device=dev)
+            eng.gemm([dict(A=(a, s * xr), B=(f, s * xr), C=(ds, s * xr * xr), m=xr, n=xr,
+                           k=xl, lda=d * xr, ldb=d * xr, ldc=xr, flags=nat.GEMM_CONJ_A)
+                      for s in range(d)])
+            dsv.append(ds)
+            if i + 1 < n:
+                env = torch.empty((xr, xr), dtype=c64, device=dev)
+                eng.gemm([dict(A=a, B=f, C=env, m=xr, n=xr, k=xl * d, lda=xr, ldb=xr, ldc=xr,
+                               flags=nat.GEMM_CONJ_A)])
+        p = torch.empty((n, d), dtype=c64, device=dev)
+        # p[i, s] = conj(Q_{i+1})^T . vec(D_s): a 1 x 1 product over xr^2 terms
+        eng.gemm([dict(A=q[i + 1], B=(dsv[i], s * q[i + 1].numel()), C=(p, i * d + s), m=1, n=1,
+                       k=q[i + 1].numel(), lda=1, ldb=1, ldc=1, flags=nat.GEMM_CONJ_A)
+                  for i in range(n) for s in range(d)])
+        return p.real
+
+    def _sample(self, as_string: bool = False, marginals: Optional[np.ndarray] = None):
+        """core.py:654-682. The reference draws each site from the diagonal
+        of its own one-site reduced density matrix of the unmeasured state
+        (core.py:668 reads `self`, not the projected copy), so the sites of
+        one sample are drawn independently from their marginals; this keeps
+        that behaviour and the same sequence of np.random.choice calls."""
+        if marginals is None:
+            marginals = self._marginal_probabilities()
+        states = list(range(self._qudit_dimension))
+        string = [np.random.choice(states, size=1, p=marginals[i])[0]
+                  for i in range(self._nqudits)]
+        if as_string:
+            return "".join(str(bit) for bit in string)
+        return string
+
+    def _marginal_probabilities(self) -> np.ndarray:
+        """float32 [n, d] marginals with rounding-level negatives set to zero
+        (a zero-probability value can come out as -1e-9 in FP32)."""
+        p = self._marginals_device().cpu().numpy().astype(np.float32)
+        p[(p < 0) & (p > -1e-6)] = 0.0
+        return p
+
+    def sample(self, nsamples: int, as_hist: bool = False, as_string: bool = False):
+        """core.py:684-721: nsamples draws; as_hist gives {bitstring: count}
+        (and implies as_string). The marginals are computed once on the
+        device, since the state does not change between draws."""
+        if not isinstance(nsamples, int):
+            raise ValueError(f"Arg nsamples should be an int but is a {type(nsamples)}.")
+        if nsamples <= 0:
+            raise ValueError(f"Arg nsamples should be positive but is {nsamples}.")
+        if as_hist:
+            as_string = True
+        marg = self._marginal_probabilities()
+        raw = [self._sample(as_string, marg) for _ in range(nsamples)]
+        if as_hist:
+            hist = {}
+            for bits in raw:
+                hist[bits] = hist.get(bits, 0) + 1
+            return hist
+        return raw
 
     # ------------------------------------------------------------ one-site
     def apply_one_qudit_gate(self, gate, node_index: int, **kwargs) -> None:

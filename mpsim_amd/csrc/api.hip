@@ -49,10 +49,11 @@ struct Resolved {
     int block, max_sweeps, check_every, inner_sweeps;
     double tol;
     int block_switch;
+    int *unconverged;
 };
 
 Resolved resolve(const mpsx_svd_opts *o) {
-    Resolved r{0, kDefaultMaxSweeps, 1, kDefaultInnerSweeps, kDefaultTol, 64};
+    Resolved r{0, kDefaultMaxSweeps, 1, kDefaultInnerSweeps, kDefaultTol, 64, nullptr};
     if (o) {
         if (o->block) r.block = o->block;
         if (o->max_sweeps) r.max_sweeps = o->max_sweeps;
@@ -60,6 +61,7 @@ Resolved resolve(const mpsx_svd_opts *o) {
         if (o->inner_sweeps) r.inner_sweeps = o->inner_sweeps;
         if (o->tol > 0) r.tol = o->tol;
         if (o->block_switch > 0) r.block_switch = o->block_switch;
+        r.unconverged = o->unconverged;
     }
     return r;
 }
@@ -95,6 +97,9 @@ template <class F>
 hipError_t timed(const char *name, hipStream_t s, F &&fn) {
     Profiler &p = prof();
     if (!p.on) return fn();
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
+        return fn();  // events inside a captured graph would not be replay-timed
     hipEvent_t a, b;
     {
         std::lock_guard<std::mutex> g(p.mu);
@@ -386,12 +391,54 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
 
     if (rows > 0) {
         HIP_TRY(timed("k_rownorm", stream, [&] { return launch_rownorm(stream, d, count, ws, rows); }), "k_rownorm");
-        HIP_TRY(timed("k_sort", stream, [&] { return launch_sort(stream, d, count, ws); }), "k_sort");
+        HIP_TRY(timed("k_sort", stream, [&] { return launch_sort(stream, d, count, ws, r.unconverged); }), "k_sort");
     }
     if (outs > 0) HIP_TRY(timed("k_gather", stream, [&] { return launch_gather(stream, d, count, ws, outs); }), "k_gather");
     if (sweeps_out) *sweeps_out = sweeps;
     if (can_poll && !converged)
         return fail(MPSX_ENOTCONV, "Jacobi SVD not converged after %d sweeps", sweeps);
+    return MPSX_OK;
+}
+
+// ---------------------------------------------------------------- graphs
+struct mpsx_graph_s {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+int mpsx_graph_begin(mpsx_stream stream) {
+    if (!stream) return fail(MPSX_EINVAL, "the null stream cannot be captured");
+    HIP_TRY(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal),
+            "hipStreamBeginCapture");
+    return MPSX_OK;
+}
+
+int mpsx_graph_end(mpsx_stream stream, mpsx_graph *out) {
+    if (!out) return fail(MPSX_EINVAL, "null output");
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamEndCapture((hipStream_t)stream, &g), "hipStreamEndCapture");
+    hipGraphExec_t e = nullptr;
+    const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    if (err != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(MPSX_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(err));
+    }
+    *out = new mpsx_graph_s{g, e};
+    return MPSX_OK;
+}
+
+int mpsx_graph_launch(mpsx_graph graph, mpsx_stream stream) {
+    if (!graph) return fail(MPSX_EINVAL, "null graph");
+    HIP_TRY(hipGraphLaunch(graph->exec, (hipStream_t)stream), "hipGraphLaunch");
+    return MPSX_OK;
+}
+
+int mpsx_graph_destroy(mpsx_graph graph) {
+    if (!graph) return MPSX_OK;
+    HIP_TRY(hipGraphExecDestroy(graph->exec), "hipGraphExecDestroy");
+    HIP_TRY(hipGraphDestroy(graph->graph), "hipGraphDestroy");
+    delete graph;
     return MPSX_OK;
 }
 

@@ -26,6 +26,7 @@ hipError_t launch_jacobi_round(hipStream_t s, int b, const mpsx_two_site_op *d, 
                                int inner_sweeps);
 hipError_t launch_sweep_end(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
                             int sweep);
-hipError_t launch_sort(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws);
+hipError_t launch_sort(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
+                       int *unconverged);
 hipError_t launch_gather(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
                          int64_t outs);

@@ -606,10 +606,11 @@ __global__ void k_sweep_end(const mpsx_two_site_op *ops, int count, char *ws, in
 
 // ---------------------------------------------------------------- finalize
 // One workgroup per problem: stable descending order of the row norms
-// (ties keep the lower row index), singular values, discarded weight.
+// (ties keep the lower row index), singular values, discarded weight, and
+// a count of problems left unconverged (when `unconverged` is not null).
 constexpr int SORT_LDS = 4096;
 __global__ void __launch_bounds__(256)
-k_sort(const mpsx_two_site_op *ops, int count, char *ws) {
+k_sort(const mpsx_two_site_op *ops, int count, char *ws, int *unconverged) {
     __shared__ double sg[SORT_LDS];
     __shared__ double red[4];
     const int pi = blockIdx.x;
@@ -637,6 +638,8 @@ k_sort(const mpsx_two_site_op *ops, int count, char *ws) {
     disc = block_sum_256(disc, red);
     __syncthreads();
     if (threadIdx.x == 0 && op.discarded) *op.discarded = disc;
+    if (threadIdx.x == 0 && unconverged && p.nb >= 2 && !ctl_ptr(ws, p)->conv)
+        atomicAdd(unconverged, 1);
     if (op.s_out)
         for (int j = threadIdx.x; j < k; j += 256) {
             const double s2 = in_lds ? sg[perm[j]] : sig[perm[j]];
@@ -731,8 +734,9 @@ hipError_t launch_sweep_end(hipStream_t s, const mpsx_two_site_op *d, int count,
     return hipGetLastError();
 }
 
-hipError_t launch_sort(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws) {
-    hipLaunchKernelGGL(k_sort, dim3((unsigned)count), dim3(256), 0, s, d, count, ws);
+hipError_t launch_sort(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
+                       int *unconverged) {
+    hipLaunchKernelGGL(k_sort, dim3((unsigned)count), dim3(256), 0, s, d, count, ws, unconverged);
     return hipGetLastError();
 }
 

@@ -15,6 +15,23 @@ import numpy as np
 
 from . import _native as nat
 
+MAX_OPS = 65535  # descriptors per grouped launch (grid y / x limits of the kernels)
+
+
+def _matrix_addr(x, rows: int, cols: int, ld: int, what: str) -> int:
+    """Device address of a rows x cols matrix with row stride `ld` stored in
+    a contiguous complex64 tensor at an element offset (x = tensor or
+    (tensor, offset)); raises if any element would fall outside it."""
+    t, off = x if isinstance(x, tuple) else (x, 0)
+    if rows == 0 or cols == 0:
+        return nat.ptr(t) + 8 * off if t.numel() else 0
+    if cols > ld or off < 0 or off + (rows - 1) * ld + cols > t.numel():
+        raise ValueError(f"gemm operand {what}: {rows}x{cols} (ld {ld}) at offset {off} "
+                         f"exceeds a tensor of {t.numel()} elements")
+    if t.element_size() != 8:
+        raise ValueError(f"gemm operand {what} must be complex64")
+    return nat.ptr(t) + 8 * off
+
 
 @dataclass
 class SvdConfig:
@@ -25,10 +42,11 @@ class SvdConfig:
     inner_sweeps: int = 0
     tol: float = 0.0
     block_switch: int = 0   # 0 = library default (64 rows)
+    unconverged: int = 0    # device address of an int32 non-convergence counter, or 0
 
     def to_c(self) -> nat.SvdOpts:
         return nat.SvdOpts(self.block, self.max_sweeps, self.check_every, self.inner_sweeps,
-                           self.tol, self.block_switch, 0)
+                           self.tol, self.block_switch, 0, self.unconverged or None)
 
 
 @dataclass
@@ -182,15 +200,27 @@ class Engine:
 
     # ------------------------------------------------------------ gemm
     def gemm(self, ops: Sequence[dict]) -> None:
-        """ops: dicts with A, B, C (torch), m, n, k, lda, ldb, ldc, flags,
-        alpha (complex)."""
+        """ops: dicts with A, B, C, m, n, k, lda, ldb, ldc, flags, alpha
+        (complex). A, B and C are complex64 tensors, or (tensor, element
+        offset) pairs for a matrix that starts inside a tensor. Every matrix
+        is checked to lie inside its tensor before anything is launched."""
         if not ops:
+            return
+        if len(ops) > MAX_OPS:  # one launch takes at most MAX_OPS products
+            for s in range(0, len(ops), MAX_OPS):
+                self.gemm(ops[s:s + MAX_OPS])
             return
         tab = np.zeros(len(ops), dtype=nat.GEMM_OP)
         for i, o in enumerate(ops):
+            m, n, k, fl = int(o["m"]), int(o["n"]), int(o["k"]), int(o.get("flags", 0))
+            # stored shapes: op(A) = A^H means A is stored k x m, else m x k
+            ra, ca = (k, m) if fl & nat.GEMM_CONJ_A else (m, k)
+            rb, cb = (n, k) if fl & nat.GEMM_CONJ_B else (k, n)
+            pa = _matrix_addr(o["A"], ra, ca, o["lda"], "A")
+            pb = _matrix_addr(o["B"], rb, cb, o["ldb"], "B")
+            pc = _matrix_addr(o["C"], m, n, o["ldc"], "C")
             al = complex(o.get("alpha", 1.0))
-            tab[i] = (nat.ptr(o["A"]), nat.ptr(o["B"]), nat.ptr(o["C"]), o["m"], o["n"], o["k"],
-                      o["lda"], o["ldb"], o["ldc"], al.real, al.imag, o.get("flags", 0), 0)
+            tab[i] = (pa, pb, pc, m, n, k, o["lda"], o["ldb"], o["ldc"], al.real, al.imag, fl, 0)
         dtab = nat.to_device_table(tab, self.dev)
         nat.check(self.lib.mpsx_gemm(self.stream, tab.ctypes.data, dtab.data_ptr(), len(ops)))
         self._last_gemm_table = dtab

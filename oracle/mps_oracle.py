@@ -378,6 +378,50 @@ class OracleMPS:
         f = (to_norm / nrm) ** (1.0 / self.n)
         self.tensors = [f * t for t in self.tensors]
 
+    def reduced_density_matrix(self, node_indices) -> np.ndarray:
+        """core.py:596-652. Validation as core.py:610-620; then a ket/bra
+        sweep from the left (core.py:622-646): at each site the bra copy's
+        tensor is the conjugate, the physical legs of traced sites are joined
+        and those of kept sites stay open; the open legs are finally ordered
+        ket legs then bra legs, each in the order node_indices gives
+        (core.py:628-632, 648), and reshaped to (d^m, d^m) (core.py:649-651)."""
+        try:
+            idx = tuple(iter(node_indices))
+        except TypeError:
+            idx = (node_indices,)
+        if len(set(idx)) < len(idx):
+            raise ValueError("Node indices contains duplicates.")
+        if min(idx) < 0 or max(idx) > self.n - 1:
+            raise IndexError("One or more invalid node indices.")
+        d = self.d
+        # env axes: [open ket legs..., open bra legs..., ket bond, bra bond]
+        env = np.ones((1, 1), dtype=np.complex128)
+        opened: List[int] = []
+        for i, a in enumerate(self.tensors):
+            a = a.astype(np.complex128)
+            m = len(opened)
+            t = np.tensordot(env, a, axes=([2 * m], [0]))        # [..., ybond, s, x']
+            if i in idx:
+                t = np.tensordot(t, np.conj(a), axes=([2 * m], [0]))  # [.., s, x', t, y']
+                # move s to the ket block end and t to the bra block end
+                order = (list(range(m)) + [2 * m] + list(range(m, 2 * m)) + [2 * m + 2]
+                         + [2 * m + 1, 2 * m + 3])
+                env = np.transpose(t, order)
+                opened.append(i)
+            else:
+                env = np.tensordot(t, np.conj(a), axes=([2 * m, 2 * m + 1], [0, 1]))
+        m = len(opened)
+        env = env.reshape([d] * (2 * m))
+        pos = [opened.index(i) for i in idx]
+        rho = np.transpose(env, pos + [m + p for p in pos])
+        return rho.reshape(d ** m, d ** m)
+
+    def marginals(self) -> np.ndarray:
+        """Diagonal of every one-site reduced density matrix, the
+        distribution core.py:667-668 draws each site from."""
+        return np.stack([self.reduced_density_matrix(i).diagonal().real
+                         for i in range(self.n)])
+
     # -- ops ------------------------------------------------------------------
     def apply_ops(self, ops: Sequence, **kwargs) -> None:
         """core.py:1221-1276: sequential dispatch by arity. Each op is

@@ -46,3 +46,22 @@ def run(ops, n: int, d: int = 2) -> np.ndarray:
     for qudits, mat in ops:
         psi = apply_gate(psi, mat, qudits, n, d)
     return psi
+
+
+def density_matrix(psi: np.ndarray, indices: Sequence[int], n: int, d: int = 2) -> np.ndarray:
+    """Partial trace of |psi><psi| keeping `indices`, in the order given:
+    rho[(s_i1 .. s_im), (t_i1 .. t_im)] = sum over the other digits of
+    psi[.. s ..] conj(psi[.. t ..]) (the convention of the density matrices
+    the reference's tests compare against, core_test.py:1635-1711)."""
+    idx = [int(i) for i in indices]
+    rest = [a for a in range(n) if a not in idx]
+    t = np.asarray(psi, dtype=np.complex128).reshape([d] * n)
+    m = np.transpose(t, idx + rest).reshape(d ** len(idx), -1)
+    return m @ m.conj().T
+
+
+def marginals(psi: np.ndarray, n: int, d: int = 2) -> np.ndarray:
+    """p[i, s]: probability of digit s on site i (squared amplitudes summed
+    over the other digits)."""
+    pr = (np.abs(np.asarray(psi)) ** 2).reshape([d] * n)
+    return np.stack([pr.sum(axis=tuple(a for a in range(n) if a != i)) for i in range(n)])

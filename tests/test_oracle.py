@@ -264,3 +264,91 @@ def test_plain_config_oracle_vs_statevector():
     m.apply_ops(c.ops)
     assert max(m.bond_dimensions()) == 256
     assert np.max(np.abs(m.wavefunction() - sv.run(c.ops, 16))) < 1e-10
+
+
+# ---------------------------------------------------------------- reduced states
+def _random_state_ops(n, seed, depth=3):
+    rng = np.random.RandomState(seed)
+    ops = [((q,), mo.haar_unitary_from_rng(2, rng)) for q in range(n)]
+    for layer in range(depth):
+        ops += [((i, i + 1), mo.haar_unitary_from_rng(4, rng)) for i in range(layer % 2, n - 1, 2)]
+    return ops
+
+
+def test_density_matrix_closed_forms():
+    """The state-vector partial trace: a product state |0>|1>|+> gives
+    |01><01| on (0, 1) (index 1) and |10><10| on (1, 0) (index 2); a Bell
+    pair gives I/2 on either qubit; keeping every site gives psi psi^H."""
+    n = 3
+    plus = np.array([1, 1]) / np.sqrt(2)
+    psi = np.kron(np.kron([1, 0], [0, 1]), plus)
+    r01 = sv.density_matrix(psi, (0, 1), n)
+    r10 = sv.density_matrix(psi, (1, 0), n)
+    assert np.allclose(r01, np.diag([0, 1, 0, 0])) and np.allclose(r10, np.diag([0, 0, 1, 0]))
+    assert np.allclose(sv.density_matrix(psi, (2,), n), np.outer(plus, plus))
+    bell = np.array([1, 0, 0, 1]) / np.sqrt(2)
+    for q in (0, 1):
+        assert np.allclose(sv.density_matrix(bell, (q,), 2), np.eye(2) / 2)
+    w = sv.run(_random_state_ops(4, 1), 4)
+    assert np.allclose(sv.density_matrix(w, range(4), 4), np.outer(w, w.conj()))
+    assert np.allclose(sv.marginals(w, 4)[2], sv.density_matrix(w, (2,), 4).diagonal().real)
+
+
+def test_rdm_known_answers():
+    """core_test.py:1588-1618: |00>, |10>, |11> one-site matrices; and the
+    index errors of core_test.py:1621-1632."""
+    zero, one = np.diag([1.0, 0.0]), np.diag([0.0, 1.0])
+    m = OracleMPS(2)
+    for i in (0, 1):
+        assert np.allclose(m.reduced_density_matrix(i), zero)
+    m.x(0)
+    assert np.allclose(m.reduced_density_matrix(0), one)
+    assert np.allclose(m.reduced_density_matrix(1), zero)
+    m.x(1)
+    assert np.allclose(m.reduced_density_matrix(0), one)
+    assert np.allclose(m.reduced_density_matrix(1), one)
+    with pytest.raises(IndexError):
+        m.reduced_density_matrix(-1)
+    with pytest.raises(IndexError):
+        m.reduced_density_matrix(22)
+    with pytest.raises(ValueError):
+        m.reduced_density_matrix([0, 0])
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_rdm_restatement_vs_statevector(d):
+    """core_test.py:1654-1711 restated: every kept set of size 1-3 of a
+    random 5-site state, in sorted and reversed order, against the partial
+    trace of the FP64 state vector; the one-site diagonals are the
+    marginals."""
+    import itertools
+    n = 5
+    rng = np.random.RandomState(7 + d)
+    ops = [((q,), mo.haar_unitary_from_rng(d, rng)) for q in range(n)]
+    for layer in range(3):
+        ops += [((i, i + 1), mo.haar_unitary_from_rng(d * d, rng))
+                for i in range(layer % 2, n - 1, 2)]
+    m = OracleMPS(n, d, dtype_mode="c128")
+    m.apply_ops(ops)
+    w = sv.run(ops, n, d)
+    for size in (1, 2, 3):
+        for sites in itertools.combinations(range(n), size):
+            for order in (sites, tuple(reversed(sites))):
+                got = m.reduced_density_matrix(list(order))
+                assert np.allclose(got, sv.density_matrix(w, order, n, d), atol=1e-12)
+    assert np.allclose(m.marginals(), sv.marginals(w, n, d), atol=1e-12)
+
+
+def test_rdm_planted_mistake_is_caught():
+    """Swapping the ket and bra legs (returning rho^T) or ignoring the
+    requested order must fail the state-vector comparison."""
+    n = 4
+    ops = _random_state_ops(n, 12)
+    m = OracleMPS(n, dtype_mode="c128")
+    m.apply_ops(ops)
+    w = sv.run(ops, n)
+    good = m.reduced_density_matrix([2, 0])
+    ref = sv.density_matrix(w, (2, 0), n)
+    assert np.allclose(good, ref, atol=1e-12)
+    assert not np.allclose(good.T, ref, atol=1e-3)
+    assert not np.allclose(m.reduced_density_matrix([0, 2]), ref, atol=1e-3)
