@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "2-site gate updates/s @χ=256; served circuits/s + p99 latency vs plain GPU"
 PEAK_FP32_MFMA = 157.3  # TFLOP/s, MI355X dense FP32 matrix
+PEAK_FP64_MFMA = 78.6   # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0       # GB/s
 
 
@@ -39,6 +40,8 @@ def parse():
     ap.add_argument("--cpu-updates", type=int, default=6)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--inner", type=int, default=0)
+    ap.add_argument("--fp32", action="store_true",
+                    help="FP32 Jacobi working matrix (faster, ~1e-6 relative error per update)")
     ap.add_argument("--workload", default="large",
                     choices=["large", "uneven", "served", "plain16", "qutrit", "copy", "accuracy"],
                     help="large (default, the metric line) | uneven | served | plain16 | qutrit")
@@ -57,10 +60,10 @@ def emit(line):
     print(json.dumps(line), flush=True)
 
 
-def base_line(metric, value, unit, args, ms_per_step, config, dtype="c64"):
+def base_line(metric, value, unit, args, ms_per_step, config, dtype=None):
     return {"metric": metric, "value": round(value, 4), "unit": unit, "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype or ("c64+f32" if getattr(args, "fp32", False) else "c64+f64"),
             "data": "synthetic (seeded Haar gates)", "config": config}
 
 
@@ -129,7 +132,9 @@ def run_qutrit(args, eng, svd):
     depth = args.depth or 16
     circ = brickwork_haar(48, depth, 3, maxsvals=1024, seed=4)
     flops, _, updates = circuit_work(circ)
-    svd_f, _ = svd_flops(circ, eng.svd.block_switch or 64)
+    svd_f, _ = svd_flops(circ, svd.block_switch or 64, svd.fp32)
+    kname = ("k_jacobi_round32" if svd.fp32 else "k_jacobi_round64") + "<32>"
+    peak = PEAK_FP32_MFMA if svd.fp32 else PEAK_FP64_MFMA
     dc = prepare(circ, eng)
     torch.cuda.synchronize()
     nat.profile_read(reset=True)
@@ -140,14 +145,14 @@ def run_qutrit(args, eng, svd):
     dt = time.perf_counter() - t0
     nat.profile_enable(False)
     kern = nat.profile_read(reset=True)
-    jr_ms, jr_n = kern.get("k_jacobi_round<32>", (0.0, 0))
-    ach = svd_f["k_jacobi_round<32>"] / (jr_ms * 1e-3) / 1e12 if jr_ms else 0.0
+    jr_ms, jr_n = kern.get(kname, (0.0, 0))
+    ach = svd_f[kname] / (jr_ms * 1e-3) / 1e12 if jr_ms else 0.0
     line = base_line("2-site gate updates/s, 48 qutrits chi<=1024", updates / dt, "2-site updates/s",
                      args, 1e3 * dt, {"workload": f"48 qutrits, brickwork depth {depth}, 9x9 Haar, "
                                       "maxsvals=1024", "updates": updates})
-    line["roofline"] = {"kernel": "k_jacobi_round<32>", "bound": "mfma", "achieved": round(ach, 3),
-                        "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
-                        "frac": round(ach / PEAK_FP32_MFMA, 5), "launches": jr_n}
+    line["roofline"] = {"kernel": kname, "bound": "mfma", "achieved": round(ach, 3),
+                        "peak": peak, "unit": "TFLOP/s",
+                        "frac": round(ach / peak, 5), "launches": jr_n}
     line["extra"] = {"kernel_ms": {k: round(v[0], 2) for k, v in sorted(kern.items())},
                      "bond_dimensions_max": max(dc.mps.bond_dimensions()),
                      "algorithmic_tflops_per_s": round(flops / dt / 1e12, 3)}
@@ -231,7 +236,7 @@ def run_served(args, eng, svd):
     emit(line)
 
 
-def svd_flops(circuit, switch=64):
+def svd_flops(circuit, switch=64, fp32=False):
     """Golub-Reinsch SVD flops (the SVD term of the per-update work), split by
     the Jacobi kernel that runs the update (32-row pivots when the rows to
     orthogonalise, d*chi_l, exceed `switch`), and the full per-update flops,
@@ -239,7 +244,8 @@ def svd_flops(circuit, switch=64):
     from mpsim_amd.circuits import two_site_flops
     n, d = circuit.n, circuit.d
     bonds = [1] * (n + 1)
-    svd = {"k_jacobi_round<16>": 0.0, "k_jacobi_round<32>": 0.0}
+    pre = "k_jacobi_round32" if fp32 else "k_jacobi_round64"
+    svd = {pre + "<16>": 0.0, pre + "<32>": 0.0}
     full = 0.0
     for layer in circuit.layers:
         for q, _ in layer:
@@ -251,7 +257,7 @@ def svd_flops(circuit, switch=64):
             f = two_site_flops(cl, cm, cr, d, k)
             M, N = d * cl, d * cr
             p, qq = max(M, N), min(M, N)
-            kern = "k_jacobi_round<32>" if M > switch else "k_jacobi_round<16>"
+            kern = pre + ("<32>" if M > switch else "<16>")
             svd[kern] += 4.0 * (4.0 * p * p * qq + 8.0 * p * qq * qq + 9.0 * qq ** 3)
             full += f
             bonds[i + 1] = k
@@ -283,7 +289,10 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     eng = Engine.get("cuda:0")
-    svd = SvdConfig(block=args.block, inner_sweeps=args.inner)
+    import dataclasses
+    svd = dataclasses.replace(eng.svd, block=args.block or eng.svd.block,
+                              inner_sweeps=args.inner or eng.svd.inner_sweeps, fp32=args.fp32)
+    peak = PEAK_FP32_MFMA if args.fp32 else PEAK_FP64_MFMA
     if args.workload == "copy":
         return run_copy(args, eng)
     if args.workload == "accuracy":
@@ -296,7 +305,7 @@ def main():
         return run_qutrit(args, eng, svd)
     circ = brickwork_haar(64, 20, 2, maxsvals=256, seed=2)
     flops, bytes_, updates = circuit_work(circ)
-    svd_f, _ = svd_flops(circ, eng.svd.block_switch or 64)
+    svd_f, _ = svd_flops(circ, svd.block_switch or 64, svd.fp32)
 
     dc = prepare(circ, eng)
     for _ in range(args.warmup):
@@ -320,7 +329,7 @@ def main():
     value = updates * args.steps / total
     ms_per_step = 1e3 * total / args.steps
 
-    dom = max(kern.items(), key=lambda kv: kv[1][0])[0] if kern else "k_jacobi_round<32>"
+    dom = max(kern.items(), key=lambda kv: kv[1][0])[0] if kern else ""
     jr_ms, jr_n = kern.get(dom, (0.0, 0))
     dom_f = svd_f.get(dom, 0.0) * args.steps
     achieved = dom_f / (jr_ms * 1e-3) / 1e12 if jr_ms > 0 else 0.0
@@ -329,9 +338,9 @@ def main():
         "kernel": dom,
         "bound": "mfma",
         "achieved": round(achieved, 3),
-        "peak": PEAK_FP32_MFMA,
+        "peak": peak,
         "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK_FP32_MFMA, 5),
+        "frac": round(achieved / peak, 5),
         "traffic": (traffic.get("kernels", {}).get(dom, {}).get("bytes_per_launch")
                     if traffic else None),
         "launches": jr_n,
@@ -387,10 +396,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64",
+        "dtype": "c64+f32" if args.fp32 else "c64+f64",
         "data": "synthetic (seeded Haar gates)",
         "config": {"workload": "64-qubit brickwork depth 20, Haar 4x4 gates, maxsvals=256, "
-                               "layer-batched", "qubits": 64, "depth": 20, "maxsvals": 256,
+                               "layer-batched",
+                   "arithmetic": ("complex64 site tensors and theta; Jacobi working matrix, "
+                                  "Gram and rotations in " + ("FP32" if args.fp32 else "FP64")), "qubits": 64, "depth": 20, "maxsvals": 256,
                    "updates_per_step": updates, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -89,7 +89,11 @@ int mpsx_apply_one_site(mpsx_stream stream, const mpsx_site_op *h,
  *       a_out = U[:, :k] S[:k],  b_out = V^H[:k, :]
  * The caller computes k = min(d*chi_l, d*chi_r, maxsvals) (core.py:1110-1130).
  * The SVD is a blocked one-sided Jacobi on the side that is not scaled by S,
- * so the scaled side never divides by a singular value.
+ * so the scaled side never divides by a singular value. theta is formed in
+ * complex FP32; by default the rotations are stored and applied in complex
+ * FP64 (FP64 matrix cores), which keeps each truncated update as accurate
+ * as a complex64 LAPACK SVD. With rotation_fp32 set they run in FP32
+ * (faster, about 1e-6 relative error per update).
  *
  * Plain split: with gate == NULL and d == 1 the op factorises the M x N
  * matrix stored row-major at `a` (M = chi_l, N = chi_r; `b`, chi_m unused)
@@ -111,6 +115,9 @@ typedef struct {
     int32_t LD;        /* row stride of the working matrix (elements)     */
     int32_t nb;        /* number of row blocks (even)                     */
     int32_t b;         /* rows per block                                  */
+    int32_t zbytes;    /* bytes per working-matrix element: 16 (complex
+                          FP64 rotations) or 8 (complex FP32)             */
+    int32_t pad;
 } mpsx_plan;
 
 typedef struct {
@@ -137,7 +144,8 @@ typedef struct {
     double tol;         /* relative orthogonality tolerance; 0 = default   */
     int32_t block_switch;/* with block = 0: problems with more than this many
                            rows use b = 16, the others b = 8; 0 = default 64 */
-    int32_t reserved;
+    int32_t rotation_fp32;/* 0: FP64 working matrix (default; tol default
+                           1e-7); 1: FP32 working matrix (tol default 1e-5) */
     int32_t *unconverged;/* device int or NULL: incremented once per problem
                            whose last launched sweep still rotated (the
                            device-side status when check_every = 0); the

@@ -75,7 +75,7 @@ def _ws_estimate(it: TwoSiteItem) -> int:
     cr = it.b.shape[2]
     C, R = (d * cr, d * cl) if it.absorb_left else (d * cl, d * cr)
     cp = C + 31
-    return 8 * cp * (R + cp + 4) + 16 * cp + 1024
+    return 16 * cp * (R + cp + 4) + 16 * cp + 1024  # complex FP64 working rows
 
 
 def tick(engine: Engine, dcs: Sequence[DeviceCircuit], svd: Optional[SvdConfig] = None,

@@ -41,8 +41,12 @@ int hip_fail(hipError_t e, const char *what) {
 constexpr int64_t kHeaderBytes = 1024;
 constexpr int kDefaultMaxSweeps = 30;
 constexpr int kDefaultInnerSweeps = 1;
-constexpr double kDefaultTol = 1e-5;
-constexpr double kNullRel = 1e-6;
+// Defaults by working precision. FP64 rotations converge to 1e-7 in the
+// same number of sweeps as to 1e-5 (the iteration is quadratic at the end)
+// and can follow rows down to 1e-9 of the norm; FP32 rotations stop at the
+// FP32 noise floor.
+constexpr double kDefaultTol64 = 1e-7, kDefaultTol32 = 1e-5;
+constexpr double kNullRel64 = 1e-9, kNullRel32 = 1e-6;
 constexpr int64_t kGridLimit = 2147483647LL;
 
 struct Resolved {
@@ -50,10 +54,13 @@ struct Resolved {
     double tol;
     int block_switch;
     int *unconverged;
+    bool fp32;
+    double null_rel;
+    int zbytes;
 };
 
 Resolved resolve(const mpsx_svd_opts *o) {
-    Resolved r{0, kDefaultMaxSweeps, 1, kDefaultInnerSweeps, kDefaultTol, 64, nullptr};
+    Resolved r{0, kDefaultMaxSweeps, 1, kDefaultInnerSweeps, 0.0, 64, nullptr, false, 0.0, 16};
     if (o) {
         if (o->block) r.block = o->block;
         if (o->max_sweeps) r.max_sweeps = o->max_sweeps;
@@ -62,7 +69,11 @@ Resolved resolve(const mpsx_svd_opts *o) {
         if (o->tol > 0) r.tol = o->tol;
         if (o->block_switch > 0) r.block_switch = o->block_switch;
         r.unconverged = o->unconverged;
+        r.fp32 = o->rotation_fp32 != 0;
     }
+    if (r.tol <= 0) r.tol = r.fp32 ? kDefaultTol32 : kDefaultTol64;
+    r.null_rel = r.fp32 ? kNullRel32 : kNullRel64;
+    r.zbytes = r.fp32 ? 8 : 16;
     return r;
 }
 
@@ -282,6 +293,8 @@ int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts
             p.Cp = nb * p.b;
         }
         p.LD = (int32_t)align_up((int64_t)p.R + p.Cp, 4);
+        p.zbytes = r.zbytes;
+        p.pad = 0;
         p.ws_off = off;
         p.tile_base = tiles;
         p.pair_base = pairs;
@@ -328,6 +341,8 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
             return fail(MPSX_EINVAL, "op %d: plan missing or stale (call mpsx_two_site_plan)", i);
         if (o.d < 1 || o.d > 4 || (p.b != 8 && p.b != 16))
             return fail(MPSX_EINVAL, "op %d: invalid plan", i);
+        if (p.zbytes != r.zbytes)
+            return fail(MPSX_EINVAL, "op %d: planned for the other rotation precision", i);
         need += problem_bytes(p);
         tiles += theta_tiles(o);
         pairs += p.nb / 2;
@@ -347,12 +362,13 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
     HIP_TRY(hipStreamIsCapturing(stream, &cap), "hipStreamIsCapturing");
     const bool can_poll = r.check_every > 0 && cap == hipStreamCaptureStatusNone;
 
-    if (rows > 0) HIP_TRY(timed("k_init_rows", stream, [&] { return launch_init_rows(stream, d, count, ws, rows); }), "k_init_rows");
+    const bool f32 = r.fp32;
+    if (rows > 0) HIP_TRY(timed("k_init_rows", stream, [&] { return launch_init_rows(stream, f32, d, count, ws, rows); }), "k_init_rows");
     for (int D = 1; D <= 4; ++D)
-        if (has_d[D] && tiles > 0) HIP_TRY(timed("k_theta", stream, [&] { return launch_theta(stream, D, d, count, ws, tiles); }), "k_theta");
+        if (has_d[D] && tiles > 0) HIP_TRY(timed("k_theta", stream, [&] { return launch_theta(stream, f32, D, d, count, ws, tiles); }), "k_theta");
     if (rows > 0) {
-        HIP_TRY(timed("k_rownorm", stream, [&] { return launch_rownorm(stream, d, count, ws, rows); }), "k_rownorm");
-        HIP_TRY(timed("k_fro", stream, [&] { return launch_fro(stream, d, count, ws, kNullRel); }), "k_fro");
+        HIP_TRY(timed("k_rownorm", stream, [&] { return launch_rownorm(stream, f32, d, count, ws, rows); }), "k_rownorm");
+        HIP_TRY(timed("k_fro", stream, [&] { return launch_fro(stream, d, count, ws, r.null_rel); }), "k_fro");
     }
 
     int sweeps = 0;
@@ -365,9 +381,11 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
             for (int round = 0; round < max_nb - 1; ++round)
                 for (int b : {8, 16})
                     if (has_b[b])
-                        HIP_TRY(timed(b == 8 ? "k_jacobi_round<16>" : "k_jacobi_round<32>", stream,
+                        HIP_TRY(timed(b == 8 ? (f32 ? "k_jacobi_round32<16>" : "k_jacobi_round64<16>")
+                                             : (f32 ? "k_jacobi_round32<32>" : "k_jacobi_round64<32>"),
+                                      stream,
                                       [&] {
-                                          return launch_jacobi_round(stream, b, d, owner, ws, pairs, round,
+                                          return launch_jacobi_round(stream, f32, b, d, owner, ws, pairs, round,
                                                                      r.tol, r.tol * 0.01, r.inner_sweeps);
                                       }),
                                 "k_jacobi_round");
@@ -390,10 +408,10 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
     }
 
     if (rows > 0) {
-        HIP_TRY(timed("k_rownorm", stream, [&] { return launch_rownorm(stream, d, count, ws, rows); }), "k_rownorm");
+        HIP_TRY(timed("k_rownorm", stream, [&] { return launch_rownorm(stream, f32, d, count, ws, rows); }), "k_rownorm");
         HIP_TRY(timed("k_sort", stream, [&] { return launch_sort(stream, d, count, ws, r.unconverged); }), "k_sort");
     }
-    if (outs > 0) HIP_TRY(timed("k_gather", stream, [&] { return launch_gather(stream, d, count, ws, outs); }), "k_gather");
+    if (outs > 0) HIP_TRY(timed("k_gather", stream, [&] { return launch_gather(stream, f32, d, count, ws, outs); }), "k_gather");
     if (sweeps_out) *sweeps_out = sweeps;
     if (can_poll && !converged)
         return fail(MPSX_ENOTCONV, "Jacobi SVD not converged after %d sweeps", sweeps);

@@ -112,8 +112,20 @@ MPSX_HD int64_t sig_off(const mpsx_plan &p) { return p.ws_off + ctl_bytes(); }
 MPSX_HD int64_t perm_off(const mpsx_plan &p) { return sig_off(p) + align_up(8LL * p.Cp, 256); }
 MPSX_HD int64_t z_off(const mpsx_plan &p) { return perm_off(p) + align_up(4LL * p.Cp, 256); }
 MPSX_HD int64_t problem_bytes(const mpsx_plan &p) {
-    return align_up(z_off(p) - p.ws_off + 8LL * p.Cp * p.LD, 256);
+    return align_up(z_off(p) - p.ws_off + (int64_t)p.zbytes * p.Cp * p.LD, 256);
 }
+
+// Complex element of the working matrix: float2 (FP32 rotations) or
+// double2 (FP64 rotations).
+template <class T> struct Cx;
+template <> struct Cx<float> {
+    using type = float2;
+    static MPSX_DEV float2 make(float x, float y) { return make_float2(x, y); }
+};
+template <> struct Cx<double> {
+    using type = double2;
+    static MPSX_DEV double2 make(double x, double y) { return make_double2(x, y); }
+};
 
 // Tile sizes of the theta kernel.
 constexpr int THETA_TL = 32;  // l values per tile

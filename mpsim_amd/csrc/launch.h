@@ -12,21 +12,21 @@ hipError_t launch_scale(hipStream_t s, const mpsx_scale_op *d, int count, int64_
 hipError_t launch_gemm(hipStream_t s, const mpsx_gemm_op *d, int count, int64_t max_m,
                        int64_t max_n);
 
-hipError_t launch_init_rows(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
+hipError_t launch_init_rows(hipStream_t s, bool f32, const mpsx_two_site_op *d, int count, char *ws,
                             int64_t rows);
-hipError_t launch_theta(hipStream_t s, int D, const mpsx_two_site_op *d, int count, char *ws,
+hipError_t launch_theta(hipStream_t s, bool f32, int D, const mpsx_two_site_op *d, int count, char *ws,
                         int64_t tiles);
-hipError_t launch_rownorm(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
+hipError_t launch_rownorm(hipStream_t s, bool f32, const mpsx_two_site_op *d, int count, char *ws,
                           int64_t rows);
 hipError_t launch_fro(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
                       double null_rel);
 hipError_t launch_owner(hipStream_t s, const mpsx_two_site_op *d, int count, int *owner);
-hipError_t launch_jacobi_round(hipStream_t s, int b, const mpsx_two_site_op *d, const int *owner,
+hipError_t launch_jacobi_round(hipStream_t s, bool f32, int b, const mpsx_two_site_op *d, const int *owner,
                                char *ws, int64_t pairs, int round, double tol, double tol_inner,
                                int inner_sweeps);
 hipError_t launch_sweep_end(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
                             int sweep);
 hipError_t launch_sort(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
                        int *unconverged);
-hipError_t launch_gather(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
+hipError_t launch_gather(hipStream_t s, bool f32, const mpsx_two_site_op *d, int count, char *ws,
                          int64_t outs);

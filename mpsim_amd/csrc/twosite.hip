@@ -18,10 +18,16 @@
 // divided by a singular value, and the unscaled factor W is exactly unitary
 // also for the zero singular values the reference keeps.
 //
-// Precision: theta, X and Y are complex64. Pivot Gram matrices are
-// accumulated in FP32 within 64-column chunks and in FP64 across chunks; the
-// pivot eigensolve (cyclic Jacobi on the 2b x 2b Hermitian Gram), the
-// convergence test and the row norms / discarded weight are FP64.
+// Precision: theta is formed in complex FP32 (from the complex64 site
+// tensors). The working matrix Z is complex FP64 by default: Gram matrices
+// and rotations run on the FP64 matrix cores, and each update is then as
+// accurate as a complex64 LAPACK SVD. Over the ~10 sweeps of an update
+// every row is rewritten a few hundred times, and an FP32 Z accumulates
+// about 1e-6 relative error per update from those stores and from rounding
+// the rotations; the FP32 variant (v_mfma_f32_16x16x4f32, Gram partials in
+// FP32 per wave, FP64 across waves) remains selectable as the faster mode.
+// The pivot eigensolve (cyclic Jacobi on the 2b x 2b Hermitian Gram), the
+// convergence test and the row norms / discarded weight are FP64 in both.
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -33,8 +39,9 @@
 
 MPSX_DEV const mpsx_two_site_op &op_at(const mpsx_two_site_op *ops, int i) { return ops[i]; }
 
-MPSX_DEV float2 *z_ptr(char *ws, const mpsx_plan &p) {
-    return reinterpret_cast<float2 *>(ws + z_off(p));
+template <class T>
+MPSX_DEV typename Cx<T>::type *z_ptr(char *ws, const mpsx_plan &p) {
+    return reinterpret_cast<typename Cx<T>::type *>(ws + z_off(p));
 }
 MPSX_DEV double *sig_ptr(char *ws, const mpsx_plan &p) {
     return reinterpret_cast<double *>(ws + sig_off(p));
@@ -61,8 +68,10 @@ MPSX_DEV double block_sum_256(double v, double *red) {
 // ---------------------------------------------------------------- init
 // One workgroup per padded row of every problem: Y <- identity, padded X
 // rows <- 0. Row 0 of each problem also resets its control block.
+template <class T>
 __global__ void __launch_bounds__(256)
 k_init_rows(const mpsx_two_site_op *ops, int count, char *ws) {
+    using Z2 = typename Cx<T>::type;
     if (blockIdx.x == 0 && threadIdx.x < 128)
         reinterpret_cast<TwoSiteHeader *>(ws)->active[threadIdx.x & 63] = 0;
     const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
@@ -70,11 +79,11 @@ k_init_rows(const mpsx_two_site_op *ops, int count, char *ws) {
     const mpsx_plan p = ops[pi].plan;
     const int row = (int)(blockIdx.x - p.row_base);
     if (row >= p.Cp) return;
-    float2 *z = z_ptr(ws, p) + (int64_t)row * p.LD;
+    Z2 *z = z_ptr<T>(ws, p) + (int64_t)row * p.LD;
     if (row >= p.C)
-        for (int c = threadIdx.x; c < p.R; c += 256) z[c] = make_float2(0.f, 0.f);
+        for (int c = threadIdx.x; c < p.R; c += 256) z[c] = Cx<T>::make(0, 0);
     for (int c = threadIdx.x; c < p.Cp; c += 256)
-        z[p.R + c] = make_float2(c == row ? 1.f : 0.f, 0.f);
+        z[p.R + c] = Cx<T>::make(c == row ? 1 : 0, 0);
     if (row == 0 && threadIdx.x == 0) {
         ProblemCtl *ctl = ctl_ptr(ws, p);
         ctl->fro2 = 0.0;
@@ -88,7 +97,7 @@ k_init_rows(const mpsx_two_site_op *ops, int count, char *ws) {
 // ---------------------------------------------------------------- theta
 // One workgroup per 32x32 (l, r) tile; thread (ty, tx) owns l in
 // {ty, ty+16}, r in {tx, tx+16} and all D*D (p1, p2) partial products.
-template <int D>
+template <int D, class T>
 __global__ void __launch_bounds__(256)
 k_theta(const mpsx_two_site_op *ops, int count, char *ws) {
     constexpr int TL = THETA_TL, TR = THETA_TR, KT = THETA_KT;
@@ -164,7 +173,7 @@ k_theta(const mpsx_two_site_op *ops, int count, char *ws) {
     __syncthreads();
 
     // Epilogue: theta[l,o1,o2,r] = sum G[(o1,o2),(p1,p2)] T[l,p1,p2,r]
-    float2 *Z = z_ptr(ws, p);
+    typename Cx<T>::type *Z = z_ptr<T>(ws, p);
     const int LD = p.LD;
     const bool transposed = (op.flags & 1) != 0;
 #pragma unroll
@@ -187,9 +196,9 @@ k_theta(const mpsx_two_site_op *ops, int count, char *ws) {
                     const int64_t row = (int64_t)l * D + o1;  // theta row (l, o1)
                     const int64_t col = (int64_t)o2 * chi_r + r;  // theta col (o2, r)
                     if (!transposed)
-                        Z[row * LD + col] = v;
+                        Z[row * LD + col] = Cx<T>::make(v.x, v.y);
                     else
-                        Z[col * LD + row] = v;
+                        Z[col * LD + row] = Cx<T>::make(v.x, v.y);
                 }
         }
     }
@@ -197,6 +206,7 @@ k_theta(const mpsx_two_site_op *ops, int count, char *ws) {
 
 // Plain split: one workgroup per row m of the M x N input matrix, copied
 // into X (or X^T for absorb-left problems).
+template <class T>
 __global__ void __launch_bounds__(256)
 k_load(const mpsx_two_site_op *ops, int count, char *ws) {
     const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
@@ -208,15 +218,16 @@ k_load(const mpsx_two_site_op *ops, int count, char *ws) {
     if (m >= op.chi_l) return;
     const int N = op.chi_r, LD = p.LD;
     const float2 *src = reinterpret_cast<const float2 *>(op.a) + (int64_t)m * N;
-    float2 *Z = z_ptr(ws, p);
+    typename Cx<T>::type *Z = z_ptr<T>(ws, p);
     if (!(op.flags & 1))
-        for (int n = threadIdx.x; n < N; n += 256) Z[(int64_t)m * LD + n] = src[n];
+        for (int n = threadIdx.x; n < N; n += 256) Z[(int64_t)m * LD + n] = Cx<T>::make(src[n].x, src[n].y);
     else
-        for (int n = threadIdx.x; n < N; n += 256) Z[(int64_t)n * LD + m] = src[n];
+        for (int n = threadIdx.x; n < N; n += 256) Z[(int64_t)n * LD + m] = Cx<T>::make(src[n].x, src[n].y);
 }
 
 // ---------------------------------------------------------------- norms
 // One workgroup per padded row: sig2[row] = sum_c |X[row, c]|^2 (FP64).
+template <class T>
 __global__ void __launch_bounds__(256)
 k_rownorm(const mpsx_two_site_op *ops, int count, char *ws) {
     __shared__ double red[4];
@@ -225,11 +236,11 @@ k_rownorm(const mpsx_two_site_op *ops, int count, char *ws) {
     const mpsx_plan p = ops[pi].plan;
     const int row = (int)(blockIdx.x - p.row_base);
     if (row >= p.Cp) return;
-    const float2 *z = z_ptr(ws, p) + (int64_t)row * p.LD;
+    const typename Cx<T>::type *z = z_ptr<T>(ws, p) + (int64_t)row * p.LD;
     double s = 0.0;
     if (row < p.C)
         for (int c = threadIdx.x; c < p.R; c += 256) {
-            const float2 v = z[c];
+            const auto v = z[c];
             s += (double)v.x * v.x + (double)v.y * v.y;
         }
     s = block_sum_256(s, red);
@@ -270,139 +281,16 @@ MPSX_DEV mfma_f4 mfma16(float a, float b, mfma_f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// Convergence test and eigensolve of one pivot (shared by both precisions).
+// H: the pivot's Hermitian Gram (FP64, LDS); Q (LDS, identity on entry)
+// accumulates the rotations. Returns false in every thread when no pair of
+// the pivot exceeds the tolerance (nothing to rotate).
 template <int B2>
-__global__ void __launch_bounds__(256)
-k_jacobi_round(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
-               double tol_inner, int inner_sweeps) {
+MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *prm_c,
+                          double2 *prm_s, int &s_any, ProblemCtl *ctl, double null2, double tol,
+                          double tol_inner, int inner_sweeps) {
     constexpr int b = B2 / 2;
-    constexpr int NS = 2 * B2;          // stacked real rows
-    constexpr int NT = NS / 16;         // 16-row tiles per side
-    constexpr int RG = B2 / 16;         // 16-row groups of complex pivot rows
-
-    // gp: FP32 partial Grams of waves (0+2) and (1+3); Q reuses the space
-    // once H has been formed.
-    __shared__ union {
-        float gp[2][NS][NS];
-        double2 q[B2][B2 + 1];
-    } sm;
-    __shared__ double2 H[B2][B2 + 1];
-    __shared__ int rows[B2];
-    __shared__ double prm_c[b];
-    __shared__ double2 prm_s[b];
-    __shared__ int s_any;
-    auto &Q = sm.q;
-
-    const int pi = owner[blockIdx.x];
-    const mpsx_plan p = ops[pi].plan;
-    if (p.b != b) return;
-    if (round >= p.nb - 1) return;
-    ProblemCtl *ctl = ctl_ptr(ws, p);
-    if (ctl->conv) return;
-    const double null2 = ctl->null2;
-    const int pair = (int)(blockIdx.x - p.pair_base);
-    int I, J;
-    rr_pair(pair, round, p.nb, &I, &J);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int l16 = lane & 15, lq = lane >> 4;
-    auto row_of = [&](int i) { return i < b ? I * b + i : J * b + (i - b); };
-    if (tid < B2) rows[tid] = row_of(tid);
-    float2 *Z = z_ptr(ws, p);
-    const int LD = p.LD, R = p.R;
-
-    // ---- Gram G = S S^T on the matrix cores, S = [Re Xp; Im Xp]. Wave w
-    // takes the 16-column blocks w, w+4, ... of X; lane (l16, lq) loads four
-    // consecutive columns of each of its complex rows g*16 + l16 and feeds
-    // column c0 + 4 lq + s at k-step s (the sum over columns is
-    // order-free; A and B operands use the same order). Operands go from
-    // global memory straight to registers: no staging, no barriers.
-    mfma_f4 acc[NT][NT];
-#pragma unroll
-    for (int u = 0; u < NT; ++u)
-#pragma unroll
-        for (int v = 0; v < NT; ++v) acc[u][v] = mfma_f4{0.f, 0.f, 0.f, 0.f};
-    int64_t goff[RG];
-#pragma unroll
-    for (int g = 0; g < RG; ++g) goff[g] = (int64_t)row_of(g * 16 + l16) * LD;
-    for (int c0 = wave * 16; c0 < R; c0 += 64) {
-        float2 v[RG][4];
-        const int cq = c0 + 4 * lq;
-        if (cq + 4 <= R) {  // 32 contiguous bytes per row: two 16-byte loads
-#pragma unroll
-            for (int g = 0; g < RG; ++g) {
-                const float4 *src = reinterpret_cast<const float4 *>(Z + goff[g] + cq);
-                const float4 lo = src[0], hi = src[1];
-                v[g][0] = make_float2(lo.x, lo.y);
-                v[g][1] = make_float2(lo.z, lo.w);
-                v[g][2] = make_float2(hi.x, hi.y);
-                v[g][3] = make_float2(hi.z, hi.w);
-            }
-        } else {
-#pragma unroll
-            for (int g = 0; g < RG; ++g)
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int c = cq + k;
-                    v[g][k] = c < R ? Z[goff[g] + c] : make_float2(0.f, 0.f);
-                }
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            float op[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) op[t] = t < NT / 2 ? v[t][k].x : v[t - NT / 2][k].y;
-#pragma unroll
-            for (int u = 0; u < NT; ++u)
-#pragma unroll
-                for (int w = 0; w < NT; ++w) acc[u][w] = mfma16(op[u], op[w], acc[u][w]);
-        }
-    }
-    // deterministic reduction of the four wave partials: (w2 + w0), (w3 + w1)
-    // in FP32, then the two halves in FP64 while forming H.
-    if (wave >= 2) {
-#pragma unroll
-        for (int u = 0; u < NT; ++u)
-#pragma unroll
-            for (int w = 0; w < NT; ++w)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) sm.gp[wave - 2][u * 16 + 4 * lq + r][w * 16 + l16] = acc[u][w][r];
-    }
-    __syncthreads();
-    if (wave < 2) {
-#pragma unroll
-        for (int u = 0; u < NT; ++u)
-#pragma unroll
-            for (int w = 0; w < NT; ++w)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) sm.gp[wave][u * 16 + 4 * lq + r][w * 16 + l16] += acc[u][w][r];
-    }
-    __syncthreads();
-    double2 hv[(B2 * B2 + 255) / 256];
-#pragma unroll
-    for (int t = 0; t < (B2 * B2 + 255) / 256; ++t) {
-        const int e = tid + 256 * t;
-        if (e < B2 * B2) {
-            const int i = e / B2, j = e % B2;
-            const int u = i <= j ? i : j, v = i <= j ? j : i;  // upper triangle, mirrored
-            auto G = [&](int r, int c) { return (double)sm.gp[0][r][c] + (double)sm.gp[1][r][c]; };
-            const double re = G(u, v) + G(B2 + u, B2 + v);
-            double im = G(B2 + u, v) - G(u, B2 + v);
-            if (i > j) im = -im;
-            if (i == j) im = 0.0;
-            hv[t] = make_double2(re, im);
-        }
-    }
-    __syncthreads();  // gp is dead from here on; Q reuses it
-#pragma unroll
-    for (int t = 0; t < (B2 * B2 + 255) / 256; ++t) {
-        const int e = tid + 256 * t;
-        if (e < B2 * B2) {
-            const int i = e / B2, j = e % B2;
-            H[i][j] = hv[t];
-            Q[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
-        }
-    }
-    __syncthreads();
-
+    const int tid = threadIdx.x;
     // ---- convergence test on every off-diagonal pair of the pivot (the
     // blocks' own rows included: a block is only orthogonal internally once
     // it has been part of a rotated pivot)
@@ -415,7 +303,7 @@ k_jacobi_round(const mpsx_two_site_op *ops, const int *owner, char *ws, int roun
         const double g2 = h.x * h.x + h.y * h.y;
         if (hi > null2 && hj > null2 && g2 > tol * tol * hi * hj) rot = 1;
     }
-    if (!__syncthreads_or(rot)) return;
+    if (!__syncthreads_or(rot)) return false;
     if (tid == 0) atomicAdd(&ctl->counter, 1);
 
     // ---- pivot eigensolve: cyclic Jacobi on H (FP64), Q accumulates.
@@ -519,6 +407,145 @@ k_jacobi_round(const mpsx_two_site_op *ops, const int *owner, char *ws, int roun
         if (!s_any) break;
     }
 
+    return true;
+}
+
+template <int B2>
+__global__ void __launch_bounds__(256)
+k_jacobi_round32(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
+               double tol_inner, int inner_sweeps) {
+    constexpr int b = B2 / 2;
+    constexpr int NS = 2 * B2;          // stacked real rows
+    constexpr int NT = NS / 16;         // 16-row tiles per side
+    constexpr int RG = B2 / 16;         // 16-row groups of complex pivot rows
+
+    // gp: FP32 partial Grams of waves (0+2) and (1+3); Q reuses the space
+    // once H has been formed.
+    __shared__ union {
+        float gp[2][NS][NS];
+        double2 q[B2][B2 + 1];
+    } sm;
+    __shared__ double2 H[B2][B2 + 1];
+    __shared__ int rows[B2];
+    __shared__ double prm_c[b];
+    __shared__ double2 prm_s[b];
+    __shared__ int s_any;
+    auto &Q = sm.q;
+
+    const int pi = owner[blockIdx.x];
+    const mpsx_plan p = ops[pi].plan;
+    if (p.b != b) return;
+    if (round >= p.nb - 1) return;
+    ProblemCtl *ctl = ctl_ptr(ws, p);
+    if (ctl->conv) return;
+    const double null2 = ctl->null2;
+    const int pair = (int)(blockIdx.x - p.pair_base);
+    int I, J;
+    rr_pair(pair, round, p.nb, &I, &J);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    auto row_of = [&](int i) { return i < b ? I * b + i : J * b + (i - b); };
+    if (tid < B2) rows[tid] = row_of(tid);
+    float2 *Z = z_ptr<float>(ws, p);
+    const int LD = p.LD, R = p.R;
+
+    // ---- Gram G = S S^T on the matrix cores, S = [Re Xp; Im Xp]. Wave w
+    // takes the 16-column blocks w, w+4, ... of X; lane (l16, lq) loads four
+    // consecutive columns of each of its complex rows g*16 + l16 and feeds
+    // column c0 + 4 lq + s at k-step s (the sum over columns is
+    // order-free; A and B operands use the same order). Operands go from
+    // global memory straight to registers: no staging, no barriers.
+    mfma_f4 acc[NT][NT];
+#pragma unroll
+    for (int u = 0; u < NT; ++u)
+#pragma unroll
+        for (int v = 0; v < NT; ++v) acc[u][v] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+    int64_t goff[RG];
+#pragma unroll
+    for (int g = 0; g < RG; ++g) goff[g] = (int64_t)row_of(g * 16 + l16) * LD;
+    for (int c0 = wave * 16; c0 < R; c0 += 64) {
+        float2 v[RG][4];
+        const int cq = c0 + 4 * lq;
+        if (cq + 4 <= R) {  // 32 contiguous bytes per row: two 16-byte loads
+#pragma unroll
+            for (int g = 0; g < RG; ++g) {
+                const float4 *src = reinterpret_cast<const float4 *>(Z + goff[g] + cq);
+                const float4 lo = src[0], hi = src[1];
+                v[g][0] = make_float2(lo.x, lo.y);
+                v[g][1] = make_float2(lo.z, lo.w);
+                v[g][2] = make_float2(hi.x, hi.y);
+                v[g][3] = make_float2(hi.z, hi.w);
+            }
+        } else {
+#pragma unroll
+            for (int g = 0; g < RG; ++g)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int c = cq + k;
+                    v[g][k] = c < R ? Z[goff[g] + c] : make_float2(0.f, 0.f);
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float op[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) op[t] = t < NT / 2 ? v[t][k].x : v[t - NT / 2][k].y;
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+#pragma unroll
+                for (int w = 0; w < NT; ++w) acc[u][w] = mfma16(op[u], op[w], acc[u][w]);
+        }
+    }
+    // deterministic reduction of the four wave partials: (w2 + w0), (w3 + w1)
+    // in FP32, then the two halves in FP64 while forming H.
+    if (wave >= 2) {
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+#pragma unroll
+            for (int w = 0; w < NT; ++w)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sm.gp[wave - 2][u * 16 + 4 * lq + r][w * 16 + l16] = acc[u][w][r];
+    }
+    __syncthreads();
+    if (wave < 2) {
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+#pragma unroll
+            for (int w = 0; w < NT; ++w)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sm.gp[wave][u * 16 + 4 * lq + r][w * 16 + l16] += acc[u][w][r];
+    }
+    __syncthreads();
+    double2 hv[(B2 * B2 + 255) / 256];
+#pragma unroll
+    for (int t = 0; t < (B2 * B2 + 255) / 256; ++t) {
+        const int e = tid + 256 * t;
+        if (e < B2 * B2) {
+            const int i = e / B2, j = e % B2;
+            const int u = i <= j ? i : j, v = i <= j ? j : i;  // upper triangle, mirrored
+            auto G = [&](int r, int c) { return (double)sm.gp[0][r][c] + (double)sm.gp[1][r][c]; };
+            const double re = G(u, v) + G(B2 + u, B2 + v);
+            double im = G(B2 + u, v) - G(u, B2 + v);
+            if (i > j) im = -im;
+            if (i == j) im = 0.0;
+            hv[t] = make_double2(re, im);
+        }
+    }
+    __syncthreads();  // gp is dead from here on; Q reuses it
+#pragma unroll
+    for (int t = 0; t < (B2 * B2 + 255) / 256; ++t) {
+        const int e = tid + 256 * t;
+        if (e < B2 * B2) {
+            const int i = e / B2, j = e % B2;
+            H[i][j] = hv[t];
+            Q[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        }
+    }
+    __syncthreads();
+
+    if (!pivot_solve<B2>(H, Q, prm_c, prm_s, s_any, ctl, null2, tol, tol_inner, inner_sweeps))
+        return;
+
     // ---- apply on the matrix cores. T[i][j] (i, j < NS) from Q:
     //   T[i][j] = Re Q[j][i], T[i][B2+j] = Im Q[j][i],
     //   T[B2+i][j] = -Im Q[j][i], T[B2+i][B2+j] = Re Q[j][i]   (i, j < B2)
@@ -572,6 +599,222 @@ k_jacobi_round(const mpsx_two_site_op *ops, const int *owner, char *ws, int roun
                 for (int r = 0; r < 4; ++r) {
                     const int row = tm * 16 + 4 * lq + r;  // complex row in the pivot
                     Z[(int64_t)rows[row] * LD + c] = make_float2(acc[tm][r], acc[tm + NT / 2][r]);
+                }
+        }
+#pragma unroll
+        for (int s = 0; s < B2 / 4; ++s) zin[s] = znx[s];
+    }
+}
+
+// The same round with an FP64 working matrix, on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64: operands laid out as in the FP32 16x16x4
+// instruction, lane l holding A[l%16][l/16] and B[l/16][l%16]; but result
+// register r of lane l is D[4*r + l/16][l%16]). Both products accumulate real and imaginary parts directly
+// (two MFMAs into each), which halves the accumulators and rotation
+// operands against the real embedding at the same MFMA count:
+//   Gram:  Re H = Xr Xr^T + Xi Xi^T,   Im H = Xi Xr^T - Xr Xi^T
+//   apply: Re Z' = Re(Q)^T Zr + Im(Q)^T Zi,  Im Z' = -Im(Q)^T Zr + Re(Q)^T Zi
+// Each wave accumulates a partial Gram over its columns; the four
+// partials are summed in a fixed order, (w2 + w0) + (w3 + w1), through LDS.
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+MPSX_DEV mfma_d4 mfma16d(double a, double b, mfma_d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <int B2>
+__global__ void __launch_bounds__(256)
+k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
+                 double tol_inner, int inner_sweeps) {
+    constexpr int b = B2 / 2;
+    constexpr int RG = B2 / 16;         // 16-row groups of complex pivot rows
+    constexpr int HT = RG;              // 16-row tiles of the complex Gram
+
+    // hp: partial complex Grams of waves (0+2) and (1+3); Q reuses the space
+    // once H has been formed.
+    __shared__ union {
+        double2 hp[2][B2][B2];
+        double2 q[B2][B2 + 1];
+    } sm;
+    __shared__ double2 H[B2][B2 + 1];
+    __shared__ int rows[B2];
+    __shared__ double prm_c[b];
+    __shared__ double2 prm_s[b];
+    __shared__ int s_any;
+    auto &Q = sm.q;
+
+    const int pi = owner[blockIdx.x];
+    const mpsx_plan p = ops[pi].plan;
+    if (p.b != b) return;
+    if (round >= p.nb - 1) return;
+    ProblemCtl *ctl = ctl_ptr(ws, p);
+    if (ctl->conv) return;
+    const double null2 = ctl->null2;
+    const int pair = (int)(blockIdx.x - p.pair_base);
+    int I, J;
+    rr_pair(pair, round, p.nb, &I, &J);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    auto row_of = [&](int i) { return i < b ? I * b + i : J * b + (i - b); };
+    if (tid < B2) rows[tid] = row_of(tid);
+    double2 *Z = z_ptr<double>(ws, p);
+    const int LD = p.LD, R = p.R;
+
+    // ---- Gram on the matrix cores. Wave w takes the 16-column blocks
+    // w, w+4, ... and lane (l16, lq) feeds column c0 + 4 lq + s of complex
+    // row u*16 + l16 at k-step s (as A operand for tile row u and as B
+    // operand for tile column u).
+    mfma_d4 gre[HT][HT], gim[HT][HT];
+#pragma unroll
+    for (int u = 0; u < HT; ++u)
+#pragma unroll
+        for (int w = 0; w < HT; ++w) {
+            gre[u][w] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+            gim[u][w] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+        }
+    int64_t goff[RG];
+#pragma unroll
+    for (int g = 0; g < RG; ++g) goff[g] = (int64_t)row_of(g * 16 + l16) * LD;
+    for (int c0 = wave * 16; c0 < R; c0 += 64) {
+        double2 v[RG][4];
+        const int cq = c0 + 4 * lq;
+        if (cq + 4 <= R) {
+#pragma unroll
+            for (int g = 0; g < RG; ++g)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v[g][k] = Z[goff[g] + cq + k];
+        } else {
+#pragma unroll
+            for (int g = 0; g < RG; ++g)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int c = cq + k;
+                    v[g][k] = c < R ? Z[goff[g] + c] : make_double2(0.0, 0.0);
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int u = 0; u < HT; ++u)
+#pragma unroll
+                for (int w = 0; w < HT; ++w) {
+                    const double ar = v[u][k].x, ai = v[u][k].y, br = v[w][k].x, bi = v[w][k].y;
+                    gre[u][w] = mfma16d(ar, br, gre[u][w]);
+                    gre[u][w] = mfma16d(ai, bi, gre[u][w]);
+                    gim[u][w] = mfma16d(ai, br, gim[u][w]);
+                    gim[u][w] = mfma16d(-ar, bi, gim[u][w]);
+                }
+    }
+    // this wave's partial H[16 u + 4 r + lq][16 w + l16]
+    double2 ph[HT][HT][4];
+#pragma unroll
+    for (int u = 0; u < HT; ++u)
+#pragma unroll
+        for (int w = 0; w < HT; ++w)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) ph[u][w][r] = make_double2(gre[u][w][r], gim[u][w][r]);
+    if (wave >= 2) {
+#pragma unroll
+        for (int u = 0; u < HT; ++u)
+#pragma unroll
+            for (int w = 0; w < HT; ++w)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sm.hp[wave - 2][u * 16 + 4 * r + lq][w * 16 + l16] = ph[u][w][r];
+    }
+    __syncthreads();
+    if (wave < 2) {
+#pragma unroll
+        for (int u = 0; u < HT; ++u)
+#pragma unroll
+            for (int w = 0; w < HT; ++w)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    double2 &d = sm.hp[wave][u * 16 + 4 * r + lq][w * 16 + l16];
+                    d = make_double2(d.x + ph[u][w][r].x, d.y + ph[u][w][r].y);
+                }
+    }
+    __syncthreads();
+    double2 hv[(B2 * B2 + 255) / 256];
+#pragma unroll
+    for (int t = 0; t < (B2 * B2 + 255) / 256; ++t) {
+        const int e = tid + 256 * t;
+        if (e < B2 * B2) {
+            const int i = e / B2, j = e % B2;
+            const int u = i <= j ? i : j, v = i <= j ? j : i;  // upper triangle, mirrored
+            const double2 a0 = sm.hp[0][u][v], a1 = sm.hp[1][u][v];
+            double im = a0.y + a1.y;
+            if (i > j) im = -im;
+            if (i == j) im = 0.0;
+            hv[t] = make_double2(a0.x + a1.x, im);
+        }
+    }
+    __syncthreads();  // hp is dead from here on; Q reuses it
+#pragma unroll
+    for (int t = 0; t < (B2 * B2 + 255) / 256; ++t) {
+        const int e = tid + 256 * t;
+        if (e < B2 * B2) {
+            const int i = e / B2, j = e % B2;
+            H[i][j] = hv[t];
+            Q[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        }
+    }
+    __syncthreads();
+
+    if (!pivot_solve<B2>(H, Q, prm_c, prm_s, s_any, ctl, null2, tol, tol_inner, inner_sweeps))
+        return;
+
+    // ---- apply on the matrix cores: Z' = Q^H Z for the pivot rows. Lane
+    // (l16, lq) of output tile tm at k-step s holds Q[4 s + lq][16 tm + l16]
+    // (A operand) and input row 4 s + lq at its column (B operand).
+    double qr[HT][B2 / 4], qi[HT][B2 / 4];
+#pragma unroll
+    for (int tm = 0; tm < HT; ++tm)
+#pragma unroll
+        for (int s = 0; s < B2 / 4; ++s) {
+            const double2 q = Q[4 * s + lq][16 * tm + l16];
+            qr[tm][s] = q.x;
+            qi[tm][s] = q.y;
+        }
+    const int W = p.R + p.Cp;
+    int64_t roff[B2 / 4];
+#pragma unroll
+    for (int s = 0; s < B2 / 4; ++s) roff[s] = (int64_t)rows[4 * s + lq] * LD;
+    double2 zin[B2 / 4], znx[B2 / 4];
+    {
+        const int c = wave * 16 + l16;
+#pragma unroll
+        for (int s = 0; s < B2 / 4; ++s) zin[s] = c < W ? Z[roff[s] + c] : make_double2(0.0, 0.0);
+    }
+    for (int c0 = wave * 16; c0 < W; c0 += 64) {
+        const int c = c0 + l16;
+        const bool ok = c < W;
+        const int cn = c + 64;
+#pragma unroll
+        for (int s = 0; s < B2 / 4; ++s) znx[s] = cn < W ? Z[roff[s] + cn] : make_double2(0.0, 0.0);
+        mfma_d4 ore[HT], oim[HT];
+#pragma unroll
+        for (int tm = 0; tm < HT; ++tm) {
+            ore[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+            oim[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+        }
+#pragma unroll
+        for (int s = 0; s < B2 / 4; ++s) {
+            const double xr = zin[s].x, xi = zin[s].y, nxr = -zin[s].x;
+#pragma unroll
+            for (int tm = 0; tm < HT; ++tm) {
+                ore[tm] = mfma16d(qr[tm][s], xr, ore[tm]);
+                ore[tm] = mfma16d(qi[tm][s], xi, ore[tm]);
+                oim[tm] = mfma16d(qi[tm][s], nxr, oim[tm]);
+                oim[tm] = mfma16d(qr[tm][s], xi, oim[tm]);
+            }
+        }
+        if (ok) {
+#pragma unroll
+            for (int tm = 0; tm < HT; ++tm)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = tm * 16 + 4 * r + lq;  // complex row in the pivot
+                    Z[(int64_t)rows[row] * LD + c] = make_double2(ore[tm][r], oim[tm][r]);
                 }
         }
 #pragma unroll
@@ -648,6 +891,7 @@ k_sort(const mpsx_two_site_op *ops, int count, char *ws, int *unconverged) {
 }
 
 // One workgroup per output row (d*chi_l rows of a_out, then k rows of b_out).
+template <class T>
 __global__ void __launch_bounds__(256)
 k_gather(const mpsx_two_site_op *ops, int count, char *ws) {
     const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
@@ -657,47 +901,69 @@ k_gather(const mpsx_two_site_op *ops, int count, char *ws) {
     const int M = op.d * op.chi_l, N = op.d * op.chi_r, k = op.k;
     const int o = (int)(blockIdx.x - p.out_base);
     if (o >= M + k) return;
-    const float2 *Z = z_ptr(ws, p);
+    const typename Cx<T>::type *Z = z_ptr<T>(ws, p);
     const int *perm = perm_ptr(ws, p);
     const int LD = p.LD, R = p.R;
     const bool left_absorb = (op.flags & 1) != 0;
+    // complex64 outputs; the conjugate of Y is taken for the unscaled factor
+    auto f2 = [](auto v, bool conj) { return make_float2((float)v.x, conj ? -(float)v.y : (float)v.y); };
     if (o < M) {
         const int m = o;
         float2 *out = reinterpret_cast<float2 *>(op.a_out) + (int64_t)m * k;
         for (int j = threadIdx.x; j < k; j += 256) {
             const int64_t src = (int64_t)perm[j] * LD;
-            out[j] = left_absorb ? Z[src + m] : cconj(Z[src + R + m]);
+            out[j] = left_absorb ? f2(Z[src + m], false) : f2(Z[src + R + m], true);
         }
     } else {
         const int j = o - M;
         float2 *out = reinterpret_cast<float2 *>(op.b_out) + (int64_t)j * N;
         const int64_t src = (int64_t)perm[j] * LD;
         for (int n = threadIdx.x; n < N; n += 256)
-            out[n] = left_absorb ? cconj(Z[src + R + n]) : Z[src + n];
+            out[n] = left_absorb ? f2(Z[src + R + n], true) : f2(Z[src + n], false);
     }
 }
 
 // ---------------------------------------------------------------- launchers
-hipError_t launch_init_rows(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
+template <class T>
+static void launch_init_rows_t(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
+                               int64_t rows) {
+    hipLaunchKernelGGL(k_init_rows<T>, dim3((unsigned)rows), dim3(256), 0, s, d, count, ws);
+}
+hipError_t launch_init_rows(hipStream_t s, bool f32, const mpsx_two_site_op *d, int count, char *ws,
                             int64_t rows) {
-    hipLaunchKernelGGL(k_init_rows, dim3((unsigned)rows), dim3(256), 0, s, d, count, ws);
+    if (f32)
+        launch_init_rows_t<float>(s, d, count, ws, rows);
+    else
+        launch_init_rows_t<double>(s, d, count, ws, rows);
     return hipGetLastError();
 }
 
-hipError_t launch_theta(hipStream_t s, int D, const mpsx_two_site_op *d, int count, char *ws,
-                        int64_t tiles) {
+template <class T>
+static hipError_t launch_theta_t(hipStream_t s, int D, const mpsx_two_site_op *d, int count,
+                                 char *ws, int64_t tiles) {
+    const dim3 g((unsigned)tiles), blk(256);
     switch (D) {
-        case 1: hipLaunchKernelGGL(k_load, dim3((unsigned)tiles), dim3(256), 0, s, d, count, ws); break;
-        case 2: hipLaunchKernelGGL(k_theta<2>, dim3((unsigned)tiles), dim3(256), 0, s, d, count, ws); break;
-        case 3: hipLaunchKernelGGL(k_theta<3>, dim3((unsigned)tiles), dim3(256), 0, s, d, count, ws); break;
-        case 4: hipLaunchKernelGGL(k_theta<4>, dim3((unsigned)tiles), dim3(256), 0, s, d, count, ws); break;
+        case 1: hipLaunchKernelGGL(k_load<T>, g, blk, 0, s, d, count, ws); break;
+        case 2: hipLaunchKernelGGL((k_theta<2, T>), g, blk, 0, s, d, count, ws); break;
+        case 3: hipLaunchKernelGGL((k_theta<3, T>), g, blk, 0, s, d, count, ws); break;
+        case 4: hipLaunchKernelGGL((k_theta<4, T>), g, blk, 0, s, d, count, ws); break;
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
+    return hipSuccess;
+}
+hipError_t launch_theta(hipStream_t s, bool f32, int D, const mpsx_two_site_op *d, int count,
+                        char *ws, int64_t tiles) {
+    const hipError_t e = f32 ? launch_theta_t<float>(s, D, d, count, ws, tiles)
+                             : launch_theta_t<double>(s, D, d, count, ws, tiles);
+    return e != hipSuccess ? e : hipGetLastError();
 }
 
-hipError_t launch_rownorm(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int64_t rows) {
-    hipLaunchKernelGGL(k_rownorm, dim3((unsigned)rows), dim3(256), 0, s, d, count, ws);
+hipError_t launch_rownorm(hipStream_t s, bool f32, const mpsx_two_site_op *d, int count, char *ws,
+                          int64_t rows) {
+    if (f32)
+        hipLaunchKernelGGL(k_rownorm<float>, dim3((unsigned)rows), dim3(256), 0, s, d, count, ws);
+    else
+        hipLaunchKernelGGL(k_rownorm<double>, dim3((unsigned)rows), dim3(256), 0, s, d, count, ws);
     return hipGetLastError();
 }
 
@@ -711,19 +977,25 @@ hipError_t launch_owner(hipStream_t s, const mpsx_two_site_op *d, int count, int
     return hipGetLastError();
 }
 
-hipError_t launch_jacobi_round(hipStream_t s, int b, const mpsx_two_site_op *d, const int *owner,
-                               char *ws, int64_t pairs, int round, double tol, double tol_inner,
-                               int inner_sweeps) {
-    switch (b) {
-        case 8:
-            hipLaunchKernelGGL(k_jacobi_round<16>, dim3((unsigned)pairs), dim3(256), 0, s, d, owner, ws,
-                               round, tol, tol_inner, inner_sweeps);
-            break;
-        case 16:
-            hipLaunchKernelGGL(k_jacobi_round<32>, dim3((unsigned)pairs), dim3(256), 0, s, d, owner, ws,
-                               round, tol, tol_inner, inner_sweeps);
-            break;
-        default: return hipErrorInvalidValue;
+hipError_t launch_jacobi_round(hipStream_t s, bool f32, int b, const mpsx_two_site_op *d,
+                               const int *owner, char *ws, int64_t pairs, int round, double tol,
+                               double tol_inner, int inner_sweeps) {
+    const dim3 g((unsigned)pairs), blk(256);
+    if (b != 8 && b != 16) return hipErrorInvalidValue;
+    if (f32) {
+        if (b == 8)
+            hipLaunchKernelGGL(k_jacobi_round32<16>, g, blk, 0, s, d, owner, ws, round, tol, tol_inner,
+                               inner_sweeps);
+        else
+            hipLaunchKernelGGL(k_jacobi_round32<32>, g, blk, 0, s, d, owner, ws, round, tol, tol_inner,
+                               inner_sweeps);
+    } else {
+        if (b == 8)
+            hipLaunchKernelGGL(k_jacobi_round64<16>, g, blk, 0, s, d, owner, ws, round, tol, tol_inner,
+                               inner_sweeps);
+        else
+            hipLaunchKernelGGL(k_jacobi_round64<32>, g, blk, 0, s, d, owner, ws, round, tol, tol_inner,
+                               inner_sweeps);
     }
     return hipGetLastError();
 }
@@ -740,7 +1012,11 @@ hipError_t launch_sort(hipStream_t s, const mpsx_two_site_op *d, int count, char
     return hipGetLastError();
 }
 
-hipError_t launch_gather(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int64_t outs) {
-    hipLaunchKernelGGL(k_gather, dim3((unsigned)outs), dim3(256), 0, s, d, count, ws);
+hipError_t launch_gather(hipStream_t s, bool f32, const mpsx_two_site_op *d, int count, char *ws,
+                         int64_t outs) {
+    if (f32)
+        hipLaunchKernelGGL(k_gather<float>, dim3((unsigned)outs), dim3(256), 0, s, d, count, ws);
+    else
+        hipLaunchKernelGGL(k_gather<double>, dim3((unsigned)outs), dim3(256), 0, s, d, count, ws);
     return hipGetLastError();
 }

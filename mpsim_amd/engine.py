@@ -43,10 +43,12 @@ class SvdConfig:
     tol: float = 0.0
     block_switch: int = 0   # 0 = library default (64 rows)
     unconverged: int = 0    # device address of an int32 non-convergence counter, or 0
+    fp32: bool = False      # FP32 working matrix (faster, ~1e-6 relative error per update)
 
     def to_c(self) -> nat.SvdOpts:
         return nat.SvdOpts(self.block, self.max_sweeps, self.check_every, self.inner_sweeps,
-                           self.tol, self.block_switch, 0, self.unconverged or None)
+                           self.tol, self.block_switch, 1 if self.fp32 else 0,
+                           self.unconverged or None)
 
 
 @dataclass

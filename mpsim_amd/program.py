@@ -185,11 +185,11 @@ class CircuitProgram:
 
     def _opts(self, sweeps: int) -> nat.SvdOpts:
         c = self.svd
-        if sweeps:  # graph form: fixed sweep count, device-side status
-            return SvdConfig(c.block, sweeps, 0, c.inner_sweeps, c.tol, c.block_switch,
-                             int(self.unconverged.data_ptr())).to_c()
-        return SvdConfig(c.block, c.max_sweeps, 1, c.inner_sweeps, c.tol, c.block_switch,
-                         int(self.unconverged.data_ptr())).to_c()
+        # graph form (sweeps > 0): fixed sweep count, device-side status
+        return SvdConfig(block=c.block, max_sweeps=sweeps if sweeps else c.max_sweeps,
+                         check_every=0 if sweeps else 1, inner_sweeps=c.inner_sweeps, tol=c.tol,
+                         block_switch=c.block_switch, unconverged=int(self.unconverged.data_ptr()),
+                         fp32=c.fp32).to_c()
 
     def _plan(self, tab: np.ndarray) -> int:
         nbytes = ctypes.c_size_t(0)

@@ -29,7 +29,7 @@ from .engine import Engine, SvdConfig
 
 def workspace_bytes(circuit: Circuit) -> int:
     """Upper bound of the Jacobi workspace of the circuit's largest two-site
-    layer (padded rows Cp <= C + 31, row length R + Cp)."""
+    layer (padded rows Cp <= C + 31, row length R + Cp, complex FP64)."""
     d = circuit.d
     best = 0
     prev = [1] * (circuit.n + 1)
@@ -40,7 +40,7 @@ def workspace_bytes(circuit: Circuit) -> int:
                 i = min(q)
                 M, N = d * prev[i], d * prev[i + 2]
                 cp = M + 31
-                tot += 8 * cp * (N + cp + 4) + 16 * cp + 1024
+                tot += 16 * cp * (N + cp + 4) + 16 * cp + 1024  # complex FP64 rows
         best = max(best, tot)
         prev = [1] + list(bonds) + [1]
     return best

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match():
     from mpsim_amd import _native as nat
     lib = nat.load()
-    assert lib.mpsx_sizeof(1) == nat.TWO_SITE_OP.itemsize == 144
+    assert lib.mpsx_sizeof(1) == nat.TWO_SITE_OP.itemsize == 152
     assert lib.mpsx_sizeof(0) == nat.SITE_OP.itemsize
     assert lib.mpsx_sizeof(2) == nat.GEMM_OP.itemsize
     assert lib.mpsx_sizeof(3) == nat.SCALE_OP.itemsize
