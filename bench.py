@@ -183,6 +183,27 @@ def run_uneven(args, eng, svd):
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
     tb = sum(times) / len(times)
+    # the same batch as one captured schedule (CircuitProgram: first run eager,
+    # then hipGraph replays)
+    from mpsim_amd.program import CircuitProgram
+    prog = CircuitProgram(circs, eng, svd)
+    prog.run()
+    prog.run()
+    torch.cuda.synchronize()
+    tg_list = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        prog.run()
+        torch.cuda.synchronize()
+        tg_list.append(time.perf_counter() - t0)
+    tg = sum(tg_list) / len(tg_list)
+    graph_info = {"graph_circuits_per_s": round(len(circs) / tg, 4), "graph_replays": prog.replays,
+                  "graph_eager_runs": prog.eager_runs, "graph_unconverged": prog.last_unconverged,
+                  "graph_arena_gb": round(prog.arena_bytes / 2 ** 30, 3),
+                  "graph_workspace_gb": round(prog.workspace.numel() / 2 ** 30, 3)}
+    prog.close()
+    del prog
     sample = circs[:args.plain_sample]
     run_plain(sample[0], device=eng.dev)
     torch.cuda.synchronize()
@@ -199,7 +220,8 @@ def run_uneven(args, eng, svd):
     line["extra"] = {"plain_gpu_circuits_per_s": round(1.0 / tp, 4),
                      "plain_sample": f"first {len(sample)} circuits, one at a time",
                      "batched_vs_plain": round(value * tp, 3),
-                     "updates_per_s": round(updates / tb, 2)}
+                     "updates_per_s": round(updates / tb, 2), **graph_info,
+                     "graph_vs_plain": round(len(circs) / tg * tp, 3)}
     emit(line)
 
 

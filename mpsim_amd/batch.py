@@ -32,10 +32,31 @@ class DeviceCircuit:
     gates: List[List[object]]          # per layer, per op: device matrix
     t: int = 0                         # next layer
     discarded: List[object] = field(default_factory=list)
+    # per layer: [(site, gate)] one-site ops and [(i, gate, maxsvals)] pairs (i, i+1)
+    plan: List[tuple] = field(default_factory=list)
 
     @property
     def done(self) -> bool:
         return self.t >= len(self.circuit.layers)
+
+
+def _layer_plan(circuit: Circuit, gates: List[List[object]]) -> List[tuple]:
+    """Per layer: the one-site ops as (site, gate address) and the two-site
+    ops as (left site, gate); checks that every pair is nearest-neighbour
+    (i, i+1). The gate matrices stay alive in DeviceCircuit.gates."""
+    plan = []
+    for layer, glayer in zip(circuit.layers, gates):
+        one, two = [], []
+        for (q, _), g in zip(layer, glayer):
+            if len(q) == 1:
+                one.append((q[0], int(g.data_ptr())))
+            else:
+                i, j = q
+                if j != i + 1:
+                    raise ValueError("batched execution takes nearest-neighbour pairs (i, i+1)")
+                two.append((i, g))
+        plan.append((one, two))
+    return plan
 
 
 def upload_gates(engine: Engine, circuit: Circuit) -> List[List[object]]:
@@ -59,7 +80,8 @@ def upload_gates(engine: Engine, circuit: Circuit) -> List[List[object]]:
 def prepare(circuit: Circuit, engine: Optional[Engine] = None) -> DeviceCircuit:
     eng = engine or Engine.get()
     mps = MPS(circuit.n, circuit.d, device=eng.dev)
-    return DeviceCircuit(circuit, mps, upload_gates(eng, circuit))
+    gates = upload_gates(eng, circuit)
+    return DeviceCircuit(circuit, mps, gates, plan=_layer_plan(circuit, gates))
 
 
 def reset(dc: DeviceCircuit) -> None:
@@ -85,29 +107,28 @@ def tick(engine: Engine, dcs: Sequence[DeviceCircuit], svd: Optional[SvdConfig] 
     library in as few calls as the workspace budget allows."""
     one, two, owners = [], [], []
     for dc in dcs:
-        if dc.done:
+        if dc.t >= len(dc.plan):
             continue
         c = dc.circuit
-        layer, glayer = c.layers[dc.t], dc.gates[dc.t]
+        ones, twos = dc.plan[dc.t]
         tensors = dc.mps._tensors
         d = c.d
-        for (q, _), g in zip(layer, glayer):
-            if len(q) == 1:
-                t = tensors[q[0]]
-                if t.numel():
-                    one.append((t, t, g))
-            else:
-                i, j = q
-                if j != i + 1:
-                    raise ValueError("batched execution takes nearest-neighbour pairs (i, i+1)")
-                a, b = tensors[i], tensors[i + 1]
-                M, N = d * a.shape[0], d * b.shape[2]
-                k = min(M, N) if c.maxsvals is None else min(M, N, int(c.maxsvals))
-                two.append(TwoSiteItem(a, b, g, k))
-                owners.append((dc, i))
+        cap = None if c.maxsvals is None else int(c.maxsvals)
+        for q, gp in ones:
+            t = tensors[q]
+            cl, _, cr = t.shape
+            if cl and cr:  # an empty bond (maxsvals=0) leaves nothing to update
+                p = t.data_ptr()  # site tensors are contiguous by construction
+                one.append((p, p, gp, cl, cr, d, 0))
+        for i, g in twos:
+            a, b = tensors[i], tensors[i + 1]
+            M, N = d * a.shape[0], d * b.shape[2]
+            k = min(M, N) if cap is None else min(M, N, cap)
+            two.append(TwoSiteItem(a, b, g, k))
+            owners.append((dc, i))
         dc.t += 1
     for s in range(0, len(one), MAX_GROUP):
-        engine.one_site(one[s:s + MAX_GROUP])
+        engine.one_site_rows(one[s:s + MAX_GROUP])
     start = 0
     while start < len(two):
         end, acc = start, 0

@@ -307,32 +307,47 @@ MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *pr
     if (tid == 0) atomicAdd(&ctl->counter, 1);
 
     // ---- pivot eigensolve: cyclic Jacobi on H (FP64), Q accumulates.
-    // Work items of a step: the b(b+1)/2 upper 2x2 blocks of H and the B2*b
-    // (row, pair) entries of Q; a thread's items are fixed across steps and
-    // the index pairs of a step are computed arithmetically.
-    constexpr int NH = b * (b + 1) / 2, NQ = B2 * b, NIT = (NH + NQ + 255) / 256;
-    int ik1[NIT], ik2[NIT];
-#pragma unroll
-    for (int t = 0; t < NIT; ++t) {
-        const int it = tid + 256 * t;
-        ik1[t] = -1;
-        ik2[t] = -1;
-        if (it < NH) {
-            int k1 = 0, rem = it;
-            while (rem >= b - k1) { rem -= b - k1; ++k1; }
-            ik1[t] = k1;
-            ik2[t] = k1 + rem;
-        } else if (it < NH + NQ) {
-            ik1[t] = -2 - (it - NH) / b;  // Q row, encoded
-            ik2[t] = (it - NH) % b;        // pair
-        }
+    // Work items of a step: the NH = b(b+1)/2 upper 2x2 blocks of H and the
+    // NQ = B2*b (row, pair) entries of Q. They are dealt out per wave so that
+    // lanes of one wave run the same kind of item together (divergent lanes
+    // would serialise the two paths): wave w takes H blocks [w*HPW, (w+1)*HPW)
+    // in one slot (lane < HPW) and Q entries [w*NQ/4, (w+1)*NQ/4) in QS full
+    // slots. Items are fixed across steps; the index pairs of a step are
+    // computed arithmetically. An item whose rotations are all the identity
+    // (s == 0 exactly) is skipped.
+    constexpr int NH = b * (b + 1) / 2, NQ = B2 * b;
+    constexpr int HPW = (NH + 3) / 4, QPW = NQ / 4, NQI = (QPW + 63) / 64;
+    static_assert(HPW <= 64 && NQ % 4 == 0, "pivot too large for one workgroup");
+    const int lane64 = tid & 63, wv = tid >> 6;
+    int hk1 = -1, hk2 = -1;
+    if (lane64 < HPW && wv * HPW + lane64 < NH) {
+        int k1 = 0, rem = wv * HPW + lane64;
+        while (rem >= b - k1) { rem -= b - k1; ++k1; }
+        hk1 = k1;
+        hk2 = k1 + rem;
     }
+    int qrow[NQI], qk[NQI];
+#pragma unroll
+    for (int t = 0; t < NQI; ++t) {
+        const int r = t * 64 + lane64;
+        const int j = wv * QPW + r;
+        const bool ok = r < QPW;
+        qrow[t] = ok ? j / b : -1;
+        qk[t] = ok ? j % b : 0;
+    }
+    auto nz = [](double2 v) { return v.x != 0.0 || v.y != 0.0; };
+    // step_any[step & 1]: some pair of the step rotates. The parameter lanes
+    // and lane 0, which clears the other entry for the next step, are all in
+    // wave 0, so their LDS writes stay in program order.
+    __shared__ int step_any[2];
     for (int sw = 0; sw < inner_sweeps; ++sw) {
         __syncthreads();  // every wave has read s_any of the previous sweep
-        if (tid == 0) s_any = 0;
+        if (tid == 0) {
+            s_any = 0;
+            step_any[0] = 0;
+        }
         __syncthreads();
         for (int step = 0; step < B2 - 1; ++step) {
-            int step_rot = 0;
             if (tid < b) {
                 int pp, qq;
                 inner_pair(tid, step, b, &pp, &qq);
@@ -351,23 +366,25 @@ MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *pr
                     const double wc = w * cs;
                     s = make_double2(wc * h.x, wc * h.y);
                     s_any = 1;
-                    step_rot = 1;  // (cs can round to exactly 1 while s != 0)
+                    step_any[step & 1] = 1;
                 }
                 prm_c[tid] = cs;
                 prm_s[tid] = s;
             }
-            if (!__syncthreads_or(step_rot)) continue;  // every pair already diagonal
+            __syncthreads();
+            const bool any = step_any[step & 1] != 0;
+            if (tid == 0) step_any[(step + 1) & 1] = 0;
+            if (!any) continue;  // nothing to rotate in this step: H and Q unchanged
             // H <- J^H H J on 2x2 blocks (upper blocks, mirrored), Q <- Q J.
             // J = [[c, s], [-conj(s), c]] for each index pair (p, q).
-#pragma unroll
-            for (int t = 0; t < NIT; ++t) {
-                const int k1 = ik1[t], k2 = ik2[t];
-                if (k1 >= 0) {
+            if (hk1 >= 0) {
+                const int k1 = hk1, k2 = hk2;
+                const double2 s1 = prm_s[k1], s2 = prm_s[k2];
+                if (nz(s1) || nz(s2)) {
                     int p1, q1, p2, q2;
                     inner_pair(k1, step, b, &p1, &q1);
                     inner_pair(k2, step, b, &p2, &q2);
                     const double c1 = prm_c[k1], c2 = prm_c[k2];
-                    const double2 s1 = prm_s[k1], s2 = prm_s[k2];
                     const double2 h00 = H[p1][p2], h01 = H[p1][q2], h10 = H[q1][p2], h11 = H[q1][q2];
                     const double2 ms2 = make_double2(-s2.x, s2.y);  // -conj(s2)
                     const double2 m00 = zadd(zscale(h00, c2), zmul(h01, ms2));
@@ -390,17 +407,21 @@ MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *pr
                         H[p2][q1] = zconj(n10);
                         H[q2][q1] = zconj(n11);
                     }
-                } else if (k1 <= -2) {
-                    const int row = -2 - k1, k = k2;
-                    int pp, qq;
-                    inner_pair(k, step, b, &pp, &qq);
-                    const double c = prm_c[k];
-                    const double2 s = prm_s[k];
-                    const double2 qp = Q[row][pp], qv = Q[row][qq];
-                    const double2 ms = make_double2(-s.x, s.y);  // -conj(s)
-                    Q[row][pp] = zadd(zscale(qp, c), zmul(qv, ms));
-                    Q[row][qq] = zadd(zmul(qp, s), zscale(qv, c));
                 }
+            }
+#pragma unroll
+            for (int t = 0; t < NQI; ++t) {
+                if (qrow[t] < 0) continue;
+                const int row = qrow[t], k = qk[t];
+                const double2 s = prm_s[k];
+                if (!nz(s)) continue;
+                int pp, qq;
+                inner_pair(k, step, b, &pp, &qq);
+                const double c = prm_c[k];
+                const double2 qp = Q[row][pp], qv = Q[row][qq];
+                const double2 ms = make_double2(-s.x, s.y);  // -conj(s)
+                Q[row][pp] = zadd(zscale(qp, c), zmul(qv, ms));
+                Q[row][qq] = zadd(zmul(qp, s), zscale(qv, c));
             }
             __syncthreads();
         }
@@ -675,8 +696,8 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
     int64_t goff[RG];
 #pragma unroll
     for (int g = 0; g < RG; ++g) goff[g] = (int64_t)row_of(g * 16 + l16) * LD;
-    for (int c0 = wave * 16; c0 < R; c0 += 64) {
-        double2 v[RG][4];
+    // the next column block's operands are loaded before this block's MFMAs
+    auto load_block = [&](int c0, double2 (&v)[RG][4]) {
         const int cq = c0 + 4 * lq;
         if (cq + 4 <= R) {
 #pragma unroll
@@ -692,6 +713,11 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
                     v[g][k] = c < R ? Z[goff[g] + c] : make_double2(0.0, 0.0);
                 }
         }
+    };
+    double2 v[RG][4], vn[RG][4];
+    if (wave * 16 < R) load_block(wave * 16, v);
+    for (int c0 = wave * 16; c0 < R; c0 += 64) {
+        if (c0 + 64 < R) load_block(c0 + 64, vn);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -704,6 +730,10 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
                     gim[u][w] = mfma16d(ai, br, gim[u][w]);
                     gim[u][w] = mfma16d(-ar, bi, gim[u][w]);
                 }
+#pragma unroll
+        for (int g = 0; g < RG; ++g)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[g][k] = vn[g][k];
     }
     // this wave's partial H[16 u + 4 r + lq][16 w + l16]
     double2 ph[HT][HT][4];

@@ -70,12 +70,16 @@ class SplitItem:
     want_s: bool = False
 
 
-@dataclass
 class TwoSiteResult:
-    a_out: "object"
-    b_out: "object"
-    s: Optional["object"]
-    discarded: "object"    # torch float64 scalar on device
+    """Factors of one update; `discarded` is a device float64 scalar view."""
+    __slots__ = ("a_out", "b_out", "s", "_disc", "_i")
+
+    def __init__(self, a_out, b_out, s, disc, i):
+        self.a_out, self.b_out, self.s, self._disc, self._i = a_out, b_out, s, disc, i
+
+    @property
+    def discarded(self):
+        return self._disc[self._i]
 
 
 class Engine:
@@ -136,13 +140,22 @@ class Engine:
         """items: (tensor_in, tensor_out, gate_dev [d, d]); out may be in."""
         if not items:
             return
-        tab = np.zeros(len(items), dtype=nat.SITE_OP)
-        for i, (a, out, g) in enumerate(items):
+        rows = []
+        for a, out, g in items:
             cl, d, cr = a.shape
-            tab[i] = (nat.ptr(a), nat.ptr(out), nat.ptr(g), cl, cr, d, 0)
+            pa = nat.ptr(a)
+            rows.append((pa, pa if out is a else nat.ptr(out), nat.ptr(g), cl, cr, d, 0))
+        self.one_site_rows(rows)
+
+    def one_site_rows(self, rows: Sequence[tuple]) -> None:
+        """Grouped one-site launch from ready descriptor rows
+        (a, out, gate, chi_l, chi_r, d, 0) with device addresses."""
+        if not rows:
+            return
+        tab = np.array(rows, dtype=nat.SITE_OP)
         dtab = nat.to_device_table(tab, self.dev)
         nat.check(self.lib.mpsx_apply_one_site(self.stream, tab.ctypes.data, dtab.data_ptr(),
-                                               len(items)))
+                                               len(rows)))
         del dtab
 
     # ------------------------------------------------------------ two-site
@@ -152,34 +165,52 @@ class Engine:
         n = len(items)
         if n == 0:
             return []
-        tab = np.zeros(n, dtype=nat.TWO_SITE_OP)
-        outs: List[TwoSiteResult] = []
-        disc = torch.empty(n, dtype=torch.float64, device=self.dev)
-        for i, it in enumerate(items):
+        # shapes and input pointers (one pass), then every output carved out of
+        # one allocation and the descriptor table filled column by column
+        cls_, ds, cms, crs, ks, pa, pb, pg, fl = ([] for _ in range(9))
+        for it in items:
             if isinstance(it, SplitItem):
                 cl, cr = it.mat.shape
                 d, cm = 1, 0
-                a_ptr, b_ptr, g_ptr = nat.ptr(it.mat), 0, 0
+                pa.append(nat.ptr(it.mat)); pb.append(0); pg.append(0)
             else:
                 cl, d, cm = it.a.shape
                 cm2, d2, cr = it.b.shape
                 if cm != cm2 or d != d2:
                     raise ValueError(f"site tensors do not share a bond: {tuple(it.a.shape)} {tuple(it.b.shape)}")
-                a_ptr, b_ptr, g_ptr = nat.ptr(it.a), nat.ptr(it.b), nat.ptr(it.gate)
-            a_out = torch.empty((cl, d, it.k), dtype=torch.complex64, device=self.dev)
-            b_out = torch.empty((it.k, d, cr), dtype=torch.complex64, device=self.dev)
-            s = torch.empty(it.k, dtype=torch.float32, device=self.dev) if it.want_s else None
-            tab[i]["a"] = a_ptr
-            tab[i]["b"] = b_ptr
-            tab[i]["gate"] = g_ptr
-            tab[i]["a_out"] = nat.ptr(a_out)
-            tab[i]["b_out"] = nat.ptr(b_out)
-            tab[i]["s_out"] = nat.ptr(s)
-            tab[i]["discarded"] = int(disc.data_ptr()) + 8 * i
-            tab[i]["chi_l"], tab[i]["chi_m"], tab[i]["chi_r"] = cl, cm, cr
-            tab[i]["d"], tab[i]["k"] = d, it.k
-            tab[i]["flags"] = 1 if it.absorb_left else 0
-            outs.append(TwoSiteResult(a_out, b_out, s, disc[i]))
+                pa.append(nat.ptr(it.a)); pb.append(nat.ptr(it.b)); pg.append(nat.ptr(it.gate))
+            cls_.append(cl); ds.append(d); cms.append(cm); crs.append(cr); ks.append(it.k)
+            fl.append(1 if it.absorb_left else 0)
+        cl_a, d_a, cr_a, k_a = (np.asarray(x, dtype=np.int64) for x in (cls_, ds, crs, ks))
+        na = cl_a * d_a * k_a
+        nb = k_a * d_a * cr_a
+        ns = np.array([it.k if it.want_s else 0 for it in items], dtype=np.int64)
+        # one allocation per update holds a_out, b_out and S (complex64 slots at
+        # 32-element boundaries; S as float32 pairs), so a surviving site tensor
+        # pins only its own update's memory
+        la = (na + 31) // 32 * 32
+        lb = (nb + 31) // 32 * 32
+        length = la + lb + (ns + 1) // 2
+        disc = torch.empty(n, dtype=torch.float64, device=self.dev)
+        bufs = [torch.empty(max(int(m), 1), dtype=torch.complex64, device=self.dev)
+                for m in length.tolist()]
+        ptr0 = np.array([nat.ptr(b) for b in bufs], dtype=np.int64)
+        tab = np.zeros(n, dtype=nat.TWO_SITE_OP)
+        tab["a"], tab["b"], tab["gate"] = pa, pb, pg
+        tab["a_out"] = np.where(na > 0, ptr0, 0)
+        tab["b_out"] = np.where(nb > 0, ptr0 + 8 * la, 0)
+        tab["s_out"] = np.where(ns > 0, ptr0 + 8 * (la + lb), 0)
+        tab["discarded"] = int(disc.data_ptr()) + 8 * np.arange(n, dtype=np.int64)
+        tab["chi_l"], tab["chi_m"], tab["chi_r"] = cls_, cms, crs
+        tab["d"], tab["k"], tab["flags"] = ds, ks, fl
+        outs: List[TwoSiteResult] = []
+        for i, buf in enumerate(bufs):
+            cl, d, cr, k = cls_[i], ds[i], crs[i], ks[i]
+            a_out = buf.as_strided((cl, d, k), (d * k, k, 1), 0)
+            b_out = buf.as_strided((k, d, cr), (d * cr, cr, 1), int(la[i]))
+            s = (buf.view(torch.float32).as_strided((k,), (1,), 2 * int(la[i] + lb[i]))
+                 if ns[i] else None)
+            outs.append(TwoSiteResult(a_out, b_out, s, disc, i))
         cfg = (svd or self.svd).to_c()
         import ctypes
         nbytes = ctypes.c_size_t(0)
