@@ -685,14 +685,17 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
     // w, w+4, ... and lane (l16, lq) feeds column c0 + 4 lq + s of complex
     // row u*16 + l16 at k-step s (as A operand for tile row u and as B
     // operand for tile column u).
-    mfma_d4 gre[HT][HT], gim[HT][HT];
+    // upper tiles u <= w only (H is Hermitian), three real products each:
+    //   P1 = Xr Xr^T, P2 = Xi Xi^T, P3 = (Xr + Xi)(Xr - Xi)^T,
+    //   Re H = P1 + P2, Im H = P3 - P1 + P2
+    constexpr int NTL = HT * (HT + 1) / 2;
+    mfma_d4 p1[NTL], p2[NTL], p3[NTL];
 #pragma unroll
-    for (int u = 0; u < HT; ++u)
-#pragma unroll
-        for (int w = 0; w < HT; ++w) {
-            gre[u][w] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-            gim[u][w] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-        }
+    for (int t = 0; t < NTL; ++t) {
+        p1[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+        p2[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+        p3[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+    }
     int64_t goff[RG];
 #pragma unroll
     for (int g = 0; g < RG; ++g) goff[g] = (int64_t)row_of(g * 16 + l16) * LD;
@@ -719,48 +722,50 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
     for (int c0 = wave * 16; c0 < R; c0 += 64) {
         if (c0 + 64 < R) load_block(c0 + 64, vn);
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 4; ++k) {
+            int t = 0;
 #pragma unroll
             for (int u = 0; u < HT; ++u)
 #pragma unroll
-                for (int w = 0; w < HT; ++w) {
+                for (int w = u; w < HT; ++w, ++t) {
                     const double ar = v[u][k].x, ai = v[u][k].y, br = v[w][k].x, bi = v[w][k].y;
-                    gre[u][w] = mfma16d(ar, br, gre[u][w]);
-                    gre[u][w] = mfma16d(ai, bi, gre[u][w]);
-                    gim[u][w] = mfma16d(ai, br, gim[u][w]);
-                    gim[u][w] = mfma16d(-ar, bi, gim[u][w]);
+                    p1[t] = mfma16d(ar, br, p1[t]);
+                    p2[t] = mfma16d(ai, bi, p2[t]);
+                    p3[t] = mfma16d(ar + ai, br - bi, p3[t]);
                 }
+        }
 #pragma unroll
         for (int g = 0; g < RG; ++g)
 #pragma unroll
             for (int k = 0; k < 4; ++k) v[g][k] = vn[g][k];
     }
-    // this wave's partial H[16 u + 4 r + lq][16 w + l16]
-    double2 ph[HT][HT][4];
+    // this wave's partial H[16 u + 4 r + lq][16 w + l16] for the upper tiles
+    double2 ph[NTL][4];
 #pragma unroll
-    for (int u = 0; u < HT; ++u)
+    for (int t = 0; t < NTL; ++t)
 #pragma unroll
-        for (int w = 0; w < HT; ++w)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) ph[u][w][r] = make_double2(gre[u][w][r], gim[u][w][r]);
+        for (int r = 0; r < 4; ++r)
+            ph[t][r] = make_double2(p1[t][r] + p2[t][r], p3[t][r] - p1[t][r] + p2[t][r]);
     if (wave >= 2) {
+        int t = 0;
 #pragma unroll
         for (int u = 0; u < HT; ++u)
 #pragma unroll
-            for (int w = 0; w < HT; ++w)
+            for (int w = u; w < HT; ++w, ++t)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) sm.hp[wave - 2][u * 16 + 4 * r + lq][w * 16 + l16] = ph[u][w][r];
+                for (int r = 0; r < 4; ++r) sm.hp[wave - 2][u * 16 + 4 * r + lq][w * 16 + l16] = ph[t][r];
     }
     __syncthreads();
     if (wave < 2) {
+        int t = 0;
 #pragma unroll
         for (int u = 0; u < HT; ++u)
 #pragma unroll
-            for (int w = 0; w < HT; ++w)
+            for (int w = u; w < HT; ++w, ++t)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     double2 &d = sm.hp[wave][u * 16 + 4 * r + lq][w * 16 + l16];
-                    d = make_double2(d.x + ph[u][w][r].x, d.y + ph[u][w][r].y);
+                    d = make_double2(d.x + ph[t][r].x, d.y + ph[t][r].y);
                 }
     }
     __syncthreads();
@@ -821,21 +826,23 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
         const int cn = c + 64;
 #pragma unroll
         for (int s = 0; s < B2 / 4; ++s) znx[s] = cn < W ? Z[roff[s] + cn] : make_double2(0.0, 0.0);
-        mfma_d4 ore[HT], oim[HT];
+        // three real products per step: P1 = Re(Q)^T Zr, P2 = Im(Q)^T Zi,
+        // P3 = (Re Q - Im Q)^T (Zr + Zi);  Re Z' = P1 + P2, Im Z' = P3 - P1 + P2
+        mfma_d4 a1[HT], a2[HT], a3[HT];
 #pragma unroll
         for (int tm = 0; tm < HT; ++tm) {
-            ore[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-            oim[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+            a1[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+            a2[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+            a3[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
         }
 #pragma unroll
         for (int s = 0; s < B2 / 4; ++s) {
-            const double xr = zin[s].x, xi = zin[s].y, nxr = -zin[s].x;
+            const double xr = zin[s].x, xi = zin[s].y, xs = zin[s].x + zin[s].y;
 #pragma unroll
             for (int tm = 0; tm < HT; ++tm) {
-                ore[tm] = mfma16d(qr[tm][s], xr, ore[tm]);
-                ore[tm] = mfma16d(qi[tm][s], xi, ore[tm]);
-                oim[tm] = mfma16d(qi[tm][s], nxr, oim[tm]);
-                oim[tm] = mfma16d(qr[tm][s], xi, oim[tm]);
+                a1[tm] = mfma16d(qr[tm][s], xr, a1[tm]);
+                a2[tm] = mfma16d(qi[tm][s], xi, a2[tm]);
+                a3[tm] = mfma16d(qr[tm][s] - qi[tm][s], xs, a3[tm]);
             }
         }
         if (ok) {
@@ -844,7 +851,8 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = tm * 16 + 4 * r + lq;  // complex row in the pivot
-                    Z[(int64_t)rows[row] * LD + c] = make_double2(ore[tm][r], oim[tm][r]);
+                    Z[(int64_t)rows[row] * LD + c] =
+                        make_double2(a1[tm][r] + a2[tm][r], a3[tm][r] - a1[tm][r] + a2[tm][r]);
                 }
         }
 #pragma unroll
