@@ -630,13 +630,18 @@ k_jacobi_round32(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
 // The same round with an FP64 working matrix, on the FP64 matrix cores
 // (v_mfma_f64_16x16x4f64: operands laid out as in the FP32 16x16x4
 // instruction, lane l holding A[l%16][l/16] and B[l/16][l%16]; but result
-// register r of lane l is D[4*r + l/16][l%16]). Both products accumulate real and imaginary parts directly
-// (two MFMAs into each), which halves the accumulators and rotation
-// operands against the real embedding at the same MFMA count:
+// register r of lane l is D[4*r + l/16][l%16]). Both products work on the
+// real and imaginary parts directly (no real embedding, which would double
+// the accumulators and rotation operands):
 //   Gram:  Re H = Xr Xr^T + Xi Xi^T,   Im H = Xi Xr^T - Xr Xi^T
 //   apply: Re Z' = Re(Q)^T Zr + Im(Q)^T Zi,  Im Z' = -Im(Q)^T Zr + Re(Q)^T Zi
-// Each wave accumulates a partial Gram over its columns; the four
-// partials are summed in a fixed order, (w2 + w0) + (w3 + w1), through LDS.
+// FP64 matrix-core time dominates the Gram, so it computes only the tiles on
+// and above the diagonal of the Hermitian H, each from three real products
+// instead of four (Gauss's form). The apply keeps four real products per
+// complex step: three would need a third rotation operand array in
+// registers and halve the occupancy, which measured slower. Each wave
+// accumulates a partial Gram over its columns; the four partials are summed
+// in a fixed order, (w2 + w0) + (w3 + w1), through LDS.
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 
 MPSX_DEV mfma_d4 mfma16d(double a, double b, mfma_d4 c) {
@@ -826,23 +831,21 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
         const int cn = c + 64;
 #pragma unroll
         for (int s = 0; s < B2 / 4; ++s) znx[s] = cn < W ? Z[roff[s] + cn] : make_double2(0.0, 0.0);
-        // three real products per step: P1 = Re(Q)^T Zr, P2 = Im(Q)^T Zi,
-        // P3 = (Re Q - Im Q)^T (Zr + Zi);  Re Z' = P1 + P2, Im Z' = P3 - P1 + P2
-        mfma_d4 a1[HT], a2[HT], a3[HT];
+        mfma_d4 ore[HT], oim[HT];
 #pragma unroll
         for (int tm = 0; tm < HT; ++tm) {
-            a1[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-            a2[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-            a3[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+            ore[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+            oim[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
         }
 #pragma unroll
         for (int s = 0; s < B2 / 4; ++s) {
-            const double xr = zin[s].x, xi = zin[s].y, xs = zin[s].x + zin[s].y;
+            const double xr = zin[s].x, xi = zin[s].y, nxr = -zin[s].x;
 #pragma unroll
             for (int tm = 0; tm < HT; ++tm) {
-                a1[tm] = mfma16d(qr[tm][s], xr, a1[tm]);
-                a2[tm] = mfma16d(qi[tm][s], xi, a2[tm]);
-                a3[tm] = mfma16d(qr[tm][s] - qi[tm][s], xs, a3[tm]);
+                ore[tm] = mfma16d(qr[tm][s], xr, ore[tm]);
+                ore[tm] = mfma16d(qi[tm][s], xi, ore[tm]);
+                oim[tm] = mfma16d(qi[tm][s], nxr, oim[tm]);
+                oim[tm] = mfma16d(qr[tm][s], xi, oim[tm]);
             }
         }
         if (ok) {
@@ -851,8 +854,7 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = tm * 16 + 4 * r + lq;  // complex row in the pivot
-                    Z[(int64_t)rows[row] * LD + c] =
-                        make_double2(a1[tm][r] + a2[tm][r], a3[tm][r] - a1[tm][r] + a2[tm][r]);
+                    Z[(int64_t)rows[row] * LD + c] = make_double2(ore[tm][r], oim[tm][r]);
                 }
         }
 #pragma unroll
