@@ -43,7 +43,8 @@ def parse():
     ap.add_argument("--fp32", action="store_true",
                     help="FP32 Jacobi working matrix (faster, ~1e-6 relative error per update)")
     ap.add_argument("--workload", default="large",
-                    choices=["large", "uneven", "served", "plain16", "qutrit", "copy", "accuracy"],
+                    choices=["large", "uneven", "served", "plain16", "qutrit", "copy", "accuracy",
+                             "hbm"],
                     help="large (default, the metric line) | uneven | served | plain16 | qutrit")
     ap.add_argument("--circuits", type=int, default=512, help="uneven: circuits in the batch")
     ap.add_argument("--plain-sample", type=int, default=16,
@@ -120,6 +121,102 @@ def run_copy(args, eng):
     torch.cuda.synchronize()
     emit({"workload": "copy", "bytes_read_per_launch": x.numel() * 8,
           "bytes_written_per_launch": x.numel() * 8})
+
+
+def run_hbm(args, eng, svd):
+    """Achieved bandwidth (and theta flop rate) of the kernels around the SVD,
+    timed with HIP events on their launch stream (mpsx_profile_*):
+      k_one_site  grouped one-site gate over 64 tensors [1024, 2, 1024]
+                  (1 GiB, every element read and written once per launch);
+      k_gemm      the wavefunction chain of a random 26-qubit MPS (bond cap 64,
+                  2^26 complex64 output): bytes of every GEMM's operands and result;
+      k_theta     theta = G (A B) of 31 updates at chi = 256 (FP32 FMA): flops
+                  8 d^2 chi^3 + 8 d^4 chi^2 per update, bytes A, B read + theta written;
+      k_gather    S-absorb gather of the same updates: k rows of Y and of X read
+                  (FP64) and the complex64 factors written."""
+    import numpy as np
+    import torch
+    from mpsim_amd import _native as nat
+    from mpsim_amd.core import MPS
+    from mpsim_amd.engine import TwoSiteItem
+    from mpsim_amd.gates import haar_unitary
+    rng = np.random.RandomState(7)
+    out = {}
+
+    def timed(fn, name, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        nat.profile_read(reset=True)
+        nat.profile_enable(True)
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        nat.profile_enable(False)
+        k = nat.profile_read(reset=True)
+        ms, nl = k[name]
+        return ms, nl
+
+    # one-site gate
+    n, chi, d = 64, 1024, 2
+    ts = [torch.randn(chi, d, chi, dtype=torch.complex64, device=eng.dev) for _ in range(n)]
+    g = eng.gate(haar_unitary(2, rng))
+    items = [(t, t, g) for t in ts]
+    ms, nl = timed(lambda: eng.one_site(items), "k_one_site")
+    by = sum(16 * t.numel() for t in ts) + n * d * d * 8
+    out["k_one_site"] = {"bytes_per_launch": by, "avg_us": round(1e3 * ms / nl, 2),
+                         "gb_per_s": round(by / (ms / nl * 1e-3) / 1e9, 1)}
+    del ts, items
+    torch.cuda.empty_cache()
+
+    # wavefunction GEMM chain
+    nq, cap = 26, 64
+    mps = MPS(nq, 2, device=eng.dev, _empty=True)
+    bonds = [1] + [min(2 ** (i + 1), 2 ** (nq - i - 1), cap) for i in range(nq - 1)] + [1]
+    mps._tensors = [(torch.randn(bonds[i], 2, bonds[i + 1], dtype=torch.complex64, device=eng.dev)
+                     / np.sqrt(2 * bonds[i])) for i in range(nq)]
+    reps = 3
+    ms, nl = timed(lambda: mps._wavefunction_device(), "k_gemm", reps=reps)
+    by, rows = 0, 2
+    for i in range(1, nq):
+        cl, cr = bonds[i], bonds[i + 1]
+        by += 8 * (rows * cl + cl * 2 * cr + rows * 2 * cr)
+        rows *= 2
+    out["k_gemm_wavefunction"] = {"launches_per_call": nl // reps, "bytes_per_call": by,
+                                  "ms_per_call": round(ms / reps, 3),
+                                  "gb_per_s": round(by / (ms / reps * 1e-3) / 1e9, 1)}
+    del mps
+
+    # theta and gather of 31 chi=256 updates
+    chi, cnt = 256, 31
+    gate = eng.gate(haar_unitary(4, rng))
+    its = [TwoSiteItem(torch.randn(chi, 2, chi, dtype=torch.complex64, device=eng.dev) / 32.0,
+                       torch.randn(chi, 2, chi, dtype=torch.complex64, device=eng.dev) / 32.0,
+                       gate, 2 * chi) for _ in range(cnt)]
+    eng.two_site(its, svd)
+    torch.cuda.synchronize()
+    nat.profile_read(reset=True)
+    nat.profile_enable(True)
+    for _ in range(2):
+        eng.two_site(its, svd)
+    torch.cuda.synchronize()
+    nat.profile_enable(False)
+    k = nat.profile_read(reset=True)
+    zb = 8 if svd.fp32 else 16
+    M = N = 2 * chi
+    kk = 2 * chi  # the items keep every singular value
+    th_ms, th_n = k["k_theta"]
+    th_f = cnt * (8.0 * 4 * chi ** 3 + 8.0 * 16 * chi ** 2)
+    th_b = cnt * (2 * 8 * chi * 2 * chi + zb * M * N)
+    out["k_theta"] = {"problems": cnt, "avg_us": round(1e3 * th_ms / th_n, 2),
+                      "tflops": round(th_f / (th_ms / th_n * 1e-3) / 1e12, 2),
+                      "gb_per_s": round(th_b / (th_ms / th_n * 1e-3) / 1e9, 1)}
+    ga_ms, ga_n = k["k_gather"]
+    ga_b = cnt * ((zb + 8) * kk * (M + N))
+    out["k_gather"] = {"problems": cnt, "k": kk, "avg_us": round(1e3 * ga_ms / ga_n, 2),
+                       "bytes_per_launch": ga_b,
+                       "gb_per_s": round(ga_b / (ga_ms / ga_n * 1e-3) / 1e9, 1)}
+    emit({"workload": "hbm", "peak_hbm_gb_per_s": 8000, "peak_fp32_tflops": PEAK_FP32_MFMA,
+          "kernels": out})
 
 
 def run_qutrit(args, eng, svd):
@@ -317,6 +414,8 @@ def main():
     peak = PEAK_FP32_MFMA if args.fp32 else PEAK_FP64_MFMA
     if args.workload == "copy":
         return run_copy(args, eng)
+    if args.workload == "hbm":
+        return run_hbm(args, eng, svd)
     if args.workload == "accuracy":
         return run_accuracy(args, eng, svd)
     if args.workload == "uneven":

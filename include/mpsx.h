@@ -108,7 +108,7 @@ typedef struct {
     int64_t tile_base; /* exclusive prefix of theta tiles                 */
     int64_t pair_base; /* exclusive prefix of Jacobi pivot pairs          */
     int64_t row_base;  /* exclusive prefix of padded rows                 */
-    int64_t out_base;  /* exclusive prefix of output rows (d*chi_l + k)   */
+    int64_t out_base;  /* exclusive prefix of gather workgroups          */
     int32_t C;         /* rows orthogonalised (d*chi_l or d*chi_r)         */
     int32_t R;         /* row length (the other dimension)                */
     int32_t Cp;        /* C padded to nb*b                                */

@@ -245,7 +245,8 @@ int mpsx_gemm(mpsx_stream stream, const mpsx_gemm_op *h, const mpsx_gemm_op *d, 
         max_n = std::max(max_n, o.n);
     }
     if (max_m == 0 || max_n == 0) return MPSX_OK;
-    if ((max_m + 31) / 32 > 65535) return fail(MPSX_EINVAL, "m too large for one launch");
+    if (((max_m + 31) / 32) * ((max_n + 31) / 32) > 2147483647LL)
+        return fail(MPSX_EINVAL, "m x n too large for one launch");
     HIP_TRY(timed("k_gemm", (hipStream_t)stream, [&] { return launch_gemm((hipStream_t)stream, d, count, max_m, max_n); }), "k_gemm");
     return MPSX_OK;
 }
@@ -304,7 +305,7 @@ int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts
         tiles += theta_tiles(o);
         pairs += p.nb / 2;
         rows += p.Cp;
-        outs += M + o.k;
+        outs += gather_units(M, o.k);
     }
     if (tiles > kGridLimit || pairs > kGridLimit || rows > kGridLimit || outs > kGridLimit)
         return fail(MPSX_EINVAL, "batch too large for one launch");
@@ -347,7 +348,7 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
         tiles += theta_tiles(o);
         pairs += p.nb / 2;
         rows += p.Cp;
-        outs += (int64_t)o.d * o.chi_l + o.k;
+        outs += gather_units((int64_t)o.d * o.chi_l, o.k);
         max_nb = std::max(max_nb, p.nb);
         has_d[o.d] = true;
         if (p.nb > 0) has_b[p.b] = true;

@@ -127,6 +127,13 @@ template <> struct Cx<double> {
     static MPSX_DEV double2 make(double x, double y) { return make_double2(x, y); }
 };
 
+// Workgroups of the final gather of one problem: 32x32 tiles of the
+// M x k unscaled-by-column factor (read as a transpose of rows of Z), then
+// one per row of the k x N factor.
+MPSX_HD int64_t gather_units(int64_t M, int64_t k) {
+    return ((M + 31) / 32) * ((k + 31) / 32) + k;
+}
+
 // Tile sizes of the theta kernel.
 constexpr int THETA_TL = 32;  // l values per tile
 constexpr int THETA_TR = 32;  // r values per tile

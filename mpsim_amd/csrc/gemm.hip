@@ -13,11 +13,14 @@
 constexpr int GT = 32;  // output tile edge
 constexpr int GK = 16;  // k per stage
 
-__global__ void __launch_bounds__(256) k_gemm(const mpsx_gemm_op *ops) {
+// Grid: x = (m tile, n tile) folded as m_tile * gn + n_tile (so m may need
+// more than 65535 tiles, as the last steps of a wavefunction do), y = op.
+__global__ void __launch_bounds__(256) k_gemm(const mpsx_gemm_op *ops, int64_t gn) {
     __shared__ float2 As[GT][GK + 1];
     __shared__ float2 Bs[GK][GT + 1];
-    const mpsx_gemm_op op = ops[blockIdx.z];
-    const int64_t m0 = (int64_t)blockIdx.y * GT, n0 = (int64_t)blockIdx.x * GT;
+    const mpsx_gemm_op op = ops[blockIdx.y];
+    const int64_t tile = blockIdx.x;
+    const int64_t m0 = (tile / gn) * GT, n0 = (tile % gn) * GT;
     if (m0 >= op.m || n0 >= op.n) return;
     const bool ta = (op.flags & 1) != 0, tb = (op.flags & 2) != 0, accum = (op.flags & 4) != 0;
     const float2 *A = reinterpret_cast<const float2 *>(op.A);
@@ -82,8 +85,8 @@ hipError_t launch_gemm(hipStream_t s, const mpsx_gemm_op *d, int count, int64_t 
                        int64_t max_n) {
     const int64_t gm = (max_m + GT - 1) / GT, gn = (max_n + GT - 1) / GT;
     if (gm < 1 || gn < 1) return hipSuccess;
-    if (gm > 65535 || count > 65535) return hipErrorInvalidValue;
-    dim3 grid((unsigned)gn, (unsigned)gm, (unsigned)count);
-    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, d);
+    if (gm * gn > 2147483647LL || count > 65535) return hipErrorInvalidValue;
+    dim3 grid((unsigned)(gm * gn), (unsigned)count);
+    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, d, gn);
     return hipGetLastError();
 }

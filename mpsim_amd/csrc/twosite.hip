@@ -637,11 +637,11 @@ k_jacobi_round32(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
 //   apply: Re Z' = Re(Q)^T Zr + Im(Q)^T Zi,  Im Z' = -Im(Q)^T Zr + Re(Q)^T Zi
 // FP64 matrix-core time dominates the Gram, so it computes only the tiles on
 // and above the diagonal of the Hermitian H, each from three real products
-// instead of four (Gauss's form). The apply keeps four real products per
-// complex step: three would need a third rotation operand array in
-// registers and halve the occupancy, which measured slower. Each wave
-// accumulates a partial Gram over its columns; the four partials are summed
-// in a fixed order, (w2 + w0) + (w3 + w1), through LDS.
+// instead of four (Gauss's form), and so does the apply, which reads its
+// rotation operands from LDS at every column block (held in registers, the
+// three operand arrays would halve the occupancy, which measured slower).
+// Each wave accumulates a partial Gram over its columns; the four partials
+// are summed in a fixed order, (w2 + w0) + (w3 + w1), through LDS.
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 
 MPSX_DEV mfma_d4 mfma16d(double a, double b, mfma_d4 c) {
@@ -804,17 +804,10 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
         return;
 
     // ---- apply on the matrix cores: Z' = Q^H Z for the pivot rows. Lane
-    // (l16, lq) of output tile tm at k-step s holds Q[4 s + lq][16 tm + l16]
-    // (A operand) and input row 4 s + lq at its column (B operand).
-    double qr[HT][B2 / 4], qi[HT][B2 / 4];
-#pragma unroll
-    for (int tm = 0; tm < HT; ++tm)
-#pragma unroll
-        for (int s = 0; s < B2 / 4; ++s) {
-            const double2 q = Q[4 * s + lq][16 * tm + l16];
-            qr[tm][s] = q.x;
-            qi[tm][s] = q.y;
-        }
+    // (l16, lq) of output tile tm at k-step s takes Q[4 s + lq][16 tm + l16]
+    // (A operand) and input row 4 s + lq at its column (B operand); three
+    // real products per step: P1 = Re(Q)^T Zr, P2 = Im(Q)^T Zi,
+    // P3 = (Re Q - Im Q)^T (Zr + Zi);  Re Z' = P1 + P2, Im Z' = P3 - P1 + P2.
     const int W = p.R + p.Cp;
     int64_t roff[B2 / 4];
 #pragma unroll
@@ -831,21 +824,25 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
         const int cn = c + 64;
 #pragma unroll
         for (int s = 0; s < B2 / 4; ++s) znx[s] = cn < W ? Z[roff[s] + cn] : make_double2(0.0, 0.0);
-        mfma_d4 ore[HT], oim[HT];
+        // compiler fence: the rotation operands are read from LDS inside the
+        // loop (held in registers they would halve the occupancy)
+        asm volatile("" ::: "memory");
+        mfma_d4 a1[HT], a2[HT], a3[HT];
 #pragma unroll
         for (int tm = 0; tm < HT; ++tm) {
-            ore[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-            oim[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+            a1[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+            a2[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+            a3[tm] = mfma_d4{0.0, 0.0, 0.0, 0.0};
         }
 #pragma unroll
         for (int s = 0; s < B2 / 4; ++s) {
-            const double xr = zin[s].x, xi = zin[s].y, nxr = -zin[s].x;
+            const double xr = zin[s].x, xi = zin[s].y, xs = zin[s].x + zin[s].y;
 #pragma unroll
             for (int tm = 0; tm < HT; ++tm) {
-                ore[tm] = mfma16d(qr[tm][s], xr, ore[tm]);
-                ore[tm] = mfma16d(qi[tm][s], xi, ore[tm]);
-                oim[tm] = mfma16d(qi[tm][s], nxr, oim[tm]);
-                oim[tm] = mfma16d(qr[tm][s], xi, oim[tm]);
+                const double2 q = Q[4 * s + lq][16 * tm + l16];
+                a1[tm] = mfma16d(q.x, xr, a1[tm]);
+                a2[tm] = mfma16d(q.y, xi, a2[tm]);
+                a3[tm] = mfma16d(q.x - q.y, xs, a3[tm]);
             }
         }
         if (ok) {
@@ -854,7 +851,8 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = tm * 16 + 4 * r + lq;  // complex row in the pivot
-                    Z[(int64_t)rows[row] * LD + c] = make_double2(ore[tm][r], oim[tm][r]);
+                    Z[(int64_t)rows[row] * LD + c] =
+                        make_double2(a1[tm][r] + a2[tm][r], a3[tm][r] - a1[tm][r] + a2[tm][r]);
                 }
         }
 #pragma unroll
@@ -930,36 +928,54 @@ k_sort(const mpsx_two_site_op *ops, int count, char *ws, int *unconverged) {
         }
 }
 
-// One workgroup per output row (d*chi_l rows of a_out, then k rows of b_out).
+// Gather of the kept rows into the two factors (S absorbed as the rows of X
+// already carry it). The d*chi_l x k factor a_out takes one entry from each
+// kept row of Z per output row, i.e. a transpose: 32x32 tiles are staged
+// through LDS so that both the reads (32 consecutive columns of a row of Z)
+// and the writes (32 consecutive entries of a row of a_out) are coalesced.
+// The k x N factor b_out is a row copy, one workgroup per row.
+//   absorb right: a_out[m][j] = conj(Y[perm[j]][m]),  b_out[j][n] = X[perm[j]][n]
+//   absorb left:  a_out[m][j] = X[perm[j]][m],        b_out[j][n] = conj(Y[perm[j]][n])
 template <class T>
 __global__ void __launch_bounds__(256)
 k_gather(const mpsx_two_site_op *ops, int count, char *ws) {
+    using Z2 = typename Cx<T>::type;
+    __shared__ Z2 tile[32][33];
     const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
                                OP_FIELD(plan.out_base), blockIdx.x);
     const mpsx_two_site_op &op = ops[pi];
     const mpsx_plan p = op.plan;
     const int M = op.d * op.chi_l, N = op.d * op.chi_r, k = op.k;
-    const int o = (int)(blockIdx.x - p.out_base);
-    if (o >= M + k) return;
-    const typename Cx<T>::type *Z = z_ptr<T>(ws, p);
+    const int64_t u = (int64_t)blockIdx.x - p.out_base;
+    if (u >= gather_units(M, k)) return;
+    const Z2 *Z = z_ptr<T>(ws, p);
     const int *perm = perm_ptr(ws, p);
     const int LD = p.LD, R = p.R;
     const bool left_absorb = (op.flags & 1) != 0;
     // complex64 outputs; the conjugate of Y is taken for the unscaled factor
-    auto f2 = [](auto v, bool conj) { return make_float2((float)v.x, conj ? -(float)v.y : (float)v.y); };
-    if (o < M) {
-        const int m = o;
-        float2 *out = reinterpret_cast<float2 *>(op.a_out) + (int64_t)m * k;
-        for (int j = threadIdx.x; j < k; j += 256) {
-            const int64_t src = (int64_t)perm[j] * LD;
-            out[j] = left_absorb ? f2(Z[src + m], false) : f2(Z[src + R + m], true);
+    auto f2 = [](Z2 v, bool conj) { return make_float2((float)v.x, conj ? -(float)v.y : (float)v.y); };
+    const int64_t ta = ((int64_t)(M + 31) / 32) * ((k + 31) / 32);
+    if (u < ta) {
+        const int kt = (k + 31) / 32;
+        const int m0 = (int)(u / kt) * 32, j0 = (int)(u % kt) * 32;
+        const int col0 = left_absorb ? 0 : R;
+        for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+            const int jj = e >> 5, mi = e & 31;
+            const int j = j0 + jj, m = m0 + mi;
+            if (j < k && m < M) tile[jj][mi] = Z[(int64_t)perm[j] * LD + col0 + m];
+        }
+        __syncthreads();
+        float2 *out = reinterpret_cast<float2 *>(op.a_out);
+        for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+            const int mi = e >> 5, jj = e & 31;
+            const int j = j0 + jj, m = m0 + mi;
+            if (j < k && m < M) out[(int64_t)m * k + j] = f2(tile[jj][mi], !left_absorb);
         }
     } else {
-        const int j = o - M;
+        const int j = (int)(u - ta);
         float2 *out = reinterpret_cast<float2 *>(op.b_out) + (int64_t)j * N;
-        const int64_t src = (int64_t)perm[j] * LD;
-        for (int n = threadIdx.x; n < N; n += 256)
-            out[n] = left_absorb ? f2(Z[src + R + n], true) : f2(Z[src + n], false);
+        const int64_t src = (int64_t)perm[j] * LD + (left_absorb ? R : 0);
+        for (int n = threadIdx.x; n < N; n += 256) out[n] = f2(Z[src + n], left_absorb);
     }
 }
 

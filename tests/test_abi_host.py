@@ -75,7 +75,9 @@ def test_plan_layout_and_prefixes():
         assert r["LD"] >= r["R"] + r["Cp"] and r["ws_off"] % 256 == 0
     assert list(p["tile_base"]) == [0, 1, 65]
     assert list(p["row_base"]) == [0, int(p[0]["Cp"]), int(p[0]["Cp"] + p[1]["Cp"])]
-    assert list(p["out_base"]) == [0, 4, 4 + 512 + 256]
+    # gather workgroups: 32x32 tiles of the M x k factor, then one per row of the k x N one
+    units = [-(-M // 32) * -(-k // 32) + k for M, k in ((2, 2), (512, 256))]
+    assert list(p["out_base"]) == [0, units[0], units[0] + units[1]]
     assert nbytes > p[2]["ws_off"]
 
 

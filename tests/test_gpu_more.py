@@ -270,3 +270,15 @@ def test_edge_accessors():
     mps = m.MPS(10)
     for i in range(1, 9):
         assert mps.get_right_connected_edge_of(i - 1) == mps.get_left_connected_edge_of(i)
+
+
+def test_wavefunction_with_more_than_65535_row_tiles():
+    """core.py:483-500 at 23 qubits: the last GEMM of the chain has 2^22 rows
+    (131072 row tiles of 32); H on every site gives the uniform state."""
+    import mpsim_amd as m
+    n = 23
+    mps = m.MPS(n)
+    mps.h(-1)
+    psi = mps.wavefunction()
+    assert psi.shape == (2 ** n,)
+    assert np.max(np.abs(psi - 2.0 ** (-n / 2))) < 1e-6
