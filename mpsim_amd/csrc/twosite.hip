@@ -281,6 +281,33 @@ MPSX_DEV mfma_f4 mfma16(float a, float b, mfma_f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// Rotation parameters of the pivot eigensolve. They sit on the serial path
+// of every step, and the correctly rounded FP64 sqrt, division and rsqrt are
+// long instruction sequences, so both start from the hardware estimates
+// (v_rsq_f64, v_rcp_f64) refined by Newton steps:
+//  * w = sign(tau) / (|tau| + sqrt(tau^2 + g2)) only sets the angle; an
+//    error there leaves a slightly smaller off-diagonal that later sweeps
+//    remove, so one step each suffices;
+//  * cs = 1 / sqrt(1 + w^2 g2) must make [[cs, s], [-conj(s), cs]] with
+//    s = w cs h unitary to FP64 rounding (cs^2 (1 + w^2 g2) = 1), so it gets
+//    two steps (each squares the relative error).
+// Inputs are positive and normal (g2 > 0 is required by the caller).
+MPSX_DEV double rotation_w(double tau, double g2) {
+    const double x = tau * tau + g2;
+    double r = __builtin_amdgcn_rsq(x);
+    r = r * fma(-0.5 * x * r, r, 1.5);
+    const double den = fabs(tau) + x * r;
+    double inv = __builtin_amdgcn_rcp(den);
+    inv = inv * fma(-den, inv, 2.0);
+    return tau >= 0.0 ? inv : -inv;
+}
+MPSX_DEV double unit_cos(double y) {
+    double r = __builtin_amdgcn_rsq(y);
+    r = r * fma(-0.5 * y * r, r, 1.5);
+    r = r * fma(-0.5 * y * r, r, 1.5);
+    return r;
+}
+
 // Convergence test and eigensolve of one pivot (shared by both precisions).
 // H: the pivot's Hermitian Gram (FP64, LDS); Q (LDS, identity on entry)
 // accumulates the rotations. Returns false in every thread when no pair of
@@ -361,8 +388,8 @@ MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *pr
                     // w = sign(tau) / (|tau| + sqrt(tau^2 + |h|^2)), tau = (c - a)/2;
                     // s = sn e^{i phi} = (t cs / |h|) h = w cs h.
                     const double tau = 0.5 * (c - a);
-                    const double w = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(tau * tau + g2));
-                    cs = rsqrt(1.0 + w * w * g2);
+                    const double w = rotation_w(tau, g2);
+                    cs = unit_cos(1.0 + w * w * g2);
                     const double wc = w * cs;
                     s = make_double2(wc * h.x, wc * h.y);
                     s_any = 1;
