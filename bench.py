@@ -466,6 +466,8 @@ def main():
                     if traffic else None),
         "launches": jr_n,
         "avg_launch_us": round(1e3 * jr_ms / max(jr_n, 1), 3),
+        "executed_mfma_frac_counters": ((traffic or {}).get("mfma_counters", {}).get("executed_frac")
+                                        if not svd.fp32 else None),
         "algorithmic_flops_per_launch": dom_f / max(jr_n, 1),
         "algorithmic_unit": "SVD term 4(4p^2q+8pq^2+9q^3) of the updates this kernel runs",
     }
