@@ -445,15 +445,23 @@ def main():
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
     nat.profile_enable(False)
-    kern = nat.profile_read(reset=True)
+    full = nat.profile_read_full(reset=True)
+    kern = {k: (v["total_ms"], v["launches"]) for k, v in full.items()}
     total = sum(times)
     value = updates * args.steps / total
     ms_per_step = 1e3 * total / args.steps
 
     dom = max(kern.items(), key=lambda kv: kv[1][0])[0] if kern else ""
     jr_ms, jr_n = kern.get(dom, (0.0, 0))
+    jr_busy = full.get(dom, {}).get("busy_ms", 0.0)
     dom_f = svd_f.get(dom, 0.0) * args.steps
-    achieved = dom_f / (jr_ms * 1e-3) / 1e12 if jr_ms > 0 else 0.0
+    # Large two-site calls run as two concurrent halves on two streams, so the
+    # kernel's launches overlap in pairs: its throughput is its algorithmic
+    # flops over its busy time (the union of its launch intervals); the
+    # per-launch figure (flops per launch / average launch duration) is
+    # reported beside it.
+    achieved = dom_f / (jr_busy * 1e-3) / 1e12 if jr_busy > 0 else 0.0
+    per_launch = dom_f / (jr_ms * 1e-3) / 1e12 if jr_ms > 0 else 0.0
     traffic = load_traffic()
     roofline = {
         "kernel": dom,
@@ -464,6 +472,11 @@ def main():
         "frac": round(achieved / peak, 5),
         "traffic": (traffic.get("kernels", {}).get(dom, {}).get("bytes_per_launch")
                     if traffic else None),
+        "achieved_basis": "algorithmic SVD flops of the kernel's launches / its busy time "
+                          "(union of launch intervals, HIP events on the launch streams)",
+        "busy_ms": round(jr_busy, 3),
+        "concurrency": round(jr_ms / jr_busy, 3) if jr_busy > 0 else None,
+        "achieved_per_launch": round(per_launch, 3),
         "launches": jr_n,
         "avg_launch_us": round(1e3 * jr_ms / max(jr_n, 1), 3),
         "executed_mfma_frac_counters": ((traffic or {}).get("mfma_counters", {}).get("executed_frac")

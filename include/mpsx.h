@@ -206,13 +206,15 @@ int mpsx_scale(mpsx_stream stream, const mpsx_scale_op *h,
  * Kernel timing. When enabled, every kernel the library launches is
  * bracketed by HIP events recorded on the launch stream. mpsx_profile_read
  * synchronises those events and returns, per kernel name, the summed
- * duration and the launch count since the last reset. No counterpart in
- * the reference (which has no timing hooks); used by the benchmark.
+ * duration, the busy time (length of the union of its launch intervals,
+ * smaller than the sum when launches on different streams overlap) and the
+ * launch count since the last reset. No counterpart in the reference (which
+ * has no timing hooks); used by the benchmark.
  * ---------------------------------------------------------------------- */
 int mpsx_profile_enable(int on);
 /* Writes up to `cap` entries; returns the number of distinct kernels. */
-int mpsx_profile_read(char *names, int name_len, double *total_ms, int64_t *launches,
-                      int cap, int reset);
+int mpsx_profile_read(char *names, int name_len, double *total_ms, double *busy_ms,
+                      int64_t *launches, int cap, int reset);
 
 /* ------------------------------------------------------------------------
  * Graph capture of library calls. Every call above launches kernels only

@@ -93,6 +93,7 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
         lib.mpsx_scale.argtypes = [vp, vp, vp, i32]
         lib.mpsx_profile_enable.argtypes = [i32]
         lib.mpsx_profile_read.argtypes = [ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_int64), i32, i32]
         lib.mpsx_graph_begin.argtypes = [vp]
         lib.mpsx_graph_end.argtypes = [vp, ctypes.POINTER(vp)]
@@ -160,19 +161,26 @@ def profile_enable(on: bool) -> None:
     check(load().mpsx_profile_enable(1 if on else 0))
 
 
-def profile_read(reset: bool = True) -> dict:
-    """{kernel name: (total ms, launches)} since the last reset; synchronises
-    the recorded HIP events."""
+def profile_read_full(reset: bool = True) -> dict:
+    """{kernel name: {"total_ms", "busy_ms", "launches"}} since the last
+    reset; synchronises the recorded HIP events. busy_ms is the union of the
+    kernel's launch intervals (launches on two streams may overlap)."""
     lib = load()
     cap, nlen = 64, 64
     names = ctypes.create_string_buffer(cap * nlen)
     tot = (ctypes.c_double * cap)()
+    busy = (ctypes.c_double * cap)()
     cnt = (ctypes.c_int64 * cap)()
-    n = lib.mpsx_profile_read(names, nlen, tot, cnt, cap, 1 if reset else 0)
+    n = lib.mpsx_profile_read(names, nlen, tot, busy, cnt, cap, 1 if reset else 0)
     if n < 0:
         check(n)
     out = {}
     for i in range(min(n, cap)):
         nm = names.raw[i * nlen:(i + 1) * nlen].split(b"\0", 1)[0].decode()
-        out[nm] = (float(tot[i]), int(cnt[i]))
+        out[nm] = {"total_ms": float(tot[i]), "busy_ms": float(busy[i]), "launches": int(cnt[i])}
     return out
+
+
+def profile_read(reset: bool = True) -> dict:
+    """{kernel name: (total ms, launches)} since the last reset."""
+    return {k: (v["total_ms"], v["launches"]) for k, v in profile_read_full(reset).items()}

@@ -468,28 +468,51 @@ int mpsx_profile_enable(int on) {
     return MPSX_OK;
 }
 
-int mpsx_profile_read(char *names, int name_len, double *total_ms, int64_t *launches, int cap,
-                      int reset) {
+int mpsx_profile_read(char *names, int name_len, double *total_ms, double *busy_ms,
+                      int64_t *launches, int cap, int reset) {
     Profiler &p = prof();
     std::lock_guard<std::mutex> g(p.mu);
-    std::map<std::string, std::pair<double, int64_t>> agg;
+    struct Agg {
+        double total = 0.0;
+        int64_t n = 0;
+        std::vector<std::pair<double, double>> spans;  // relative to the first record
+    };
+    std::map<std::string, Agg> agg;
     for (const ProfRec &r : p.recs) {
         HIP_TRY(hipEventSynchronize(r.b), "hipEventSynchronize");
-        float ms = 0.f;
+        float ms = 0.f, t0 = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
-        auto &slot = agg[r.name];
-        slot.first += ms;
-        slot.second += 1;
+        HIP_TRY(hipEventElapsedTime(&t0, p.recs.front().a, r.a), "hipEventElapsedTime");
+        Agg &slot = agg[r.name];
+        slot.total += ms;
+        slot.n += 1;
+        slot.spans.emplace_back((double)t0, (double)t0 + ms);
     }
     int i = 0;
-    for (const auto &kv : agg) {
+    for (auto &kv : agg) {
         if (i < cap) {
             if (names && name_len > 0) {
                 std::strncpy(names + (size_t)i * name_len, kv.first.c_str(), (size_t)name_len - 1);
                 names[(size_t)i * name_len + name_len - 1] = '\0';
             }
-            if (total_ms) total_ms[i] = kv.second.first;
-            if (launches) launches[i] = kv.second.second;
+            if (total_ms) total_ms[i] = kv.second.total;
+            if (launches) launches[i] = kv.second.n;
+            if (busy_ms) {  // union of the launch intervals (launches on several streams overlap)
+                auto &sp = kv.second.spans;
+                std::sort(sp.begin(), sp.end());
+                double busy = 0.0, lo = 0.0, hi = -1.0;
+                for (const auto &iv : sp) {
+                    if (iv.first > hi) {
+                        if (hi > lo) busy += hi - lo;
+                        lo = iv.first;
+                        hi = iv.second;
+                    } else if (iv.second > hi) {
+                        hi = iv.second;
+                    }
+                }
+                if (hi > lo) busy += hi - lo;
+                busy_ms[i] = busy;
+            }
         }
         ++i;
     }

@@ -33,6 +33,28 @@ def _matrix_addr(x, rows: int, cols: int, ld: int, what: str) -> int:
     return nat.ptr(t) + 8 * off
 
 
+SPLIT_ROWS = 128  # a problem this size keeps a round busy long enough to overlap
+
+
+def _rows(it) -> int:
+    """Rows of the working matrix: the side orthogonalised (d*chi_l, or
+    d*chi_r when S is absorbed left)."""
+    if isinstance(it, SplitItem):
+        m, n = it.mat.shape
+        return n if it.absorb_left else m
+    return it.a.shape[1] * (it.b.shape[2] if it.absorb_left else it.a.shape[0])
+
+
+def _cost(it) -> float:
+    """Relative Jacobi work of one problem: rows^2 * row length."""
+    if isinstance(it, SplitItem):
+        m, n = it.mat.shape
+    else:
+        m, n = it.a.shape[0] * it.a.shape[1], it.b.shape[1] * it.b.shape[2]
+    c, r = (n, m) if it.absorb_left else (m, n)
+    return float(c) * c * r
+
+
 @dataclass
 class SvdConfig:
     """Jacobi SVD knobs (see mpsx_svd_opts in include/mpsx.h)."""
@@ -98,6 +120,10 @@ class Engine:
         from .tune import tuned_config
         self.svd = tuned_config()
         self.last_sweeps = 0
+        self.split_min = 8          # problems with >= SPLIT_ROWS rows needed to split a call
+        self._ws2 = None
+        self._helper = None         # (stream, single-thread executor) for split calls
+        self._keep: list = []       # device tables of the last call, alive until the next
 
     @classmethod
     def get(cls, dev=None) -> "Engine":
@@ -115,11 +141,18 @@ class Engine:
     def stream(self) -> int:
         return nat.stream_handle(self.dev)
 
-    def workspace(self, nbytes: int):
-        if self._ws is None or self._ws.numel() < nbytes:
-            grow = max(nbytes, 0 if self._ws is None else int(self._ws.numel() * 1.25))
-            self._ws = self.torch.empty(max(grow, 1024), dtype=self.torch.uint8, device=self.dev)
-        return self._ws
+    def workspace(self, nbytes: int, second: bool = False):
+        """Grow-only device workspace (a second one serves the helper stream of
+        split calls)."""
+        cur = self._ws2 if second else self._ws
+        if cur is None or cur.numel() < nbytes:
+            grow = max(nbytes, 0 if cur is None else int(cur.numel() * 1.25))
+            cur = self.torch.empty(max(grow, 1024), dtype=self.torch.uint8, device=self.dev)
+            if second:
+                self._ws2 = cur
+            else:
+                self._ws = cur
+        return cur
 
     def gate(self, mat: np.ndarray):
         """Device complex64 copy of a gate matrix (cached by content)."""
@@ -159,12 +192,11 @@ class Engine:
         del dtab
 
     # ------------------------------------------------------------ two-site
-    def two_site(self, items: Sequence[TwoSiteItem], svd: Optional[SvdConfig] = None
-                 ) -> List[TwoSiteResult]:
+    def _prepare_two_site(self, items: Sequence[TwoSiteItem], cfg, second: bool = False):
+        """Outputs, planned host table, device table and workspace of one
+        library call (allocations on the calling thread's current stream)."""
         torch = self.torch
         n = len(items)
-        if n == 0:
-            return []
         # shapes and input pointers (one pass), then every output carved out of
         # one allocation and the descriptor table filled column by column
         cls_, ds, cms, crs, ks, pa, pb, pg, fl = ([] for _ in range(9))
@@ -211,24 +243,86 @@ class Engine:
             s = (buf.view(torch.float32).as_strided((k,), (1,), 2 * int(la[i] + lb[i]))
                  if ns[i] else None)
             outs.append(TwoSiteResult(a_out, b_out, s, disc, i))
-        cfg = (svd or self.svd).to_c()
         import ctypes
         nbytes = ctypes.c_size_t(0)
         nat.check(self.lib.mpsx_two_site_plan(tab.ctypes.data, n, ctypes.byref(cfg),
                                               ctypes.byref(nbytes)))
-        ws = self.workspace(nbytes.value)
+        ws = self.workspace(nbytes.value, second)
         dtab = nat.to_device_table(tab, self.dev)
+        return outs, tab, dtab, ws
+
+    def _run_two_site(self, tab, dtab, ws, cfg, stream: int) -> int:
+        """The library call itself (blocking while it polls convergence);
+        returns the sweeps it ran."""
+        import ctypes
         sweeps = ctypes.c_int(0)
-        code = self.lib.mpsx_two_site(self.stream, tab.ctypes.data, dtab.data_ptr(), n,
+        code = self.lib.mpsx_two_site(stream, tab.ctypes.data, dtab.data_ptr(), len(tab),
                                       ctypes.byref(cfg), ws.data_ptr(), ws.numel(),
                                       ctypes.byref(sweeps))
-        self.last_sweeps = sweeps.value
         if code == nat.MPSX_ENOTCONV:
             warnings.warn(nat.load().mpsx_last_error().decode(), nat.NotConvergedWarning)
         else:
             nat.check(code)
-        # keep the device table alive until the stream has consumed it
-        self._last_table = dtab
+        return sweeps.value
+
+    def two_site(self, items: Sequence[TwoSiteItem], svd: Optional[SvdConfig] = None
+                 ) -> List[TwoSiteResult]:
+        """Batched two-site updates on the current stream. A call with at
+        least `split_min` problems of SPLIT_ROWS or more rows runs as two
+        halves of equal work concurrently, the second from a worker thread on
+        a helper stream: the latency-bound pivot eigensolves of one half then
+        overlap the matrix-core phases of the other. Each problem's arithmetic
+        is the same either way."""
+        n = len(items)
+        if n == 0:
+            return []
+        cfg = (svd or self.svd).to_c()
+        self._keep = []
+        big = [i for i, it in enumerate(items) if _rows(it) >= SPLIT_ROWS]
+        if len(big) < self.split_min or cfg.check_every == 0:
+            outs, tab, dtab, ws = self._prepare_two_site(items, cfg)
+            self._keep = [dtab]
+            self.last_sweeps = self._run_two_site(tab, dtab, ws, cfg, self.stream)
+            return outs
+        # two groups of equal estimated work (rows^2 * row length per problem)
+        order = sorted(range(n), key=lambda i: -_cost(items[i]))
+        grp, load = ([], []), [0.0, 0.0]
+        for i in order:
+            g = 0 if load[0] <= load[1] else 1
+            grp[g].append(i)
+            load[g] += _cost(items[i])
+        ia, ib = sorted(grp[0]), sorted(grp[1])
+        outs_a, tab_a, dtab_a, ws_a = self._prepare_two_site([items[i] for i in ia], cfg)
+        cfg_b = (svd or self.svd).to_c()
+        outs_b, tab_b, dtab_b, ws_b = self._prepare_two_site([items[i] for i in ib], cfg_b, True)
+        self._keep = [dtab_a, dtab_b]
+        torch = self.torch
+        if self._helper is None:
+            from concurrent.futures import ThreadPoolExecutor
+            self._helper = (torch.cuda.Stream(self.dev), ThreadPoolExecutor(max_workers=1))
+        hs, pool = self._helper
+        cur = torch.cuda.current_stream(self.dev)
+        fork = torch.cuda.Event()
+        fork.record(cur)
+        hs.wait_event(fork)  # inputs and outputs are ready on the helper stream
+        dev = self.dev
+
+        def run_b():
+            torch.cuda.set_device(dev)
+            return self._run_two_site(tab_b, dtab_b, ws_b, cfg_b, int(hs.cuda_stream))
+
+        fut = pool.submit(run_b)
+        sw_a = self._run_two_site(tab_a, dtab_a, ws_a, cfg, int(cur.cuda_stream))
+        sw_b = fut.result()
+        join = torch.cuda.Event()
+        join.record(hs)
+        cur.wait_event(join)
+        self.last_sweeps = max(sw_a, sw_b)
+        outs: List[TwoSiteResult] = [None] * n
+        for j, i in enumerate(ia):
+            outs[i] = outs_a[j]
+        for j, i in enumerate(ib):
+            outs[i] = outs_b[j]
         return outs
 
     # ------------------------------------------------------------ gemm
