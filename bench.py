@@ -463,6 +463,9 @@ def main():
     achieved = dom_f / (jr_busy * 1e-3) / 1e12 if jr_busy > 0 else 0.0
     per_launch = dom_f / (jr_ms * 1e-3) / 1e12 if jr_ms > 0 else 0.0
     traffic = load_traffic()
+    # executed FP64 MFMA flops per launch (SQ_INSTS_VALU_MFMA_MOPS_F64 x 512 from a
+    # counter pass) over the live busy time
+    mc = (traffic or {}).get("mfma_counters")
     roofline = {
         "kernel": dom,
         "bound": "mfma",
@@ -479,8 +482,9 @@ def main():
         "achieved_per_launch": round(per_launch, 3),
         "launches": jr_n,
         "avg_launch_us": round(1e3 * jr_ms / max(jr_n, 1), 3),
-        "executed_mfma_frac_counters": ((traffic or {}).get("mfma_counters", {}).get("executed_frac")
-                                        if not svd.fp32 else None),
+        "executed_mfma_frac_counters": (
+            round(mc["executed_flop_per_launch"] * jr_n / (jr_busy * 1e-3) / 1e12 / peak, 4)
+            if (mc and not svd.fp32 and jr_busy > 0 and mc.get("kernel") == dom) else None),
         "algorithmic_flops_per_launch": dom_f / max(jr_n, 1),
         "algorithmic_unit": "SVD term 4(4p^2q+8pq^2+9q^3) of the updates this kernel runs",
     }
