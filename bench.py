@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--cpu-updates", type=int, default=6)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--inner", type=int, default=0)
+    ap.add_argument("--svd-method", type=int, default=0,
+                    help="0: Gram method for FP64 32-row pivots (default); 1: one-sided everywhere")
+    ap.add_argument("--split-ways", type=int, default=0,
+                    help="concurrent stream groups of a large two-site call (0 = engine default)")
     ap.add_argument("--fp32", action="store_true",
                     help="FP32 Jacobi working matrix (faster, ~1e-6 relative error per update)")
     ap.add_argument("--workload", default="large",
@@ -408,9 +412,12 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     eng = Engine.get("cuda:0")
+    if args.split_ways:
+        eng.split_ways = args.split_ways
     import dataclasses
     svd = dataclasses.replace(eng.svd, block=args.block or eng.svd.block,
-                              inner_sweeps=args.inner or eng.svd.inner_sweeps, fp32=args.fp32)
+                              inner_sweeps=args.inner or eng.svd.inner_sweeps, fp32=args.fp32,
+                              method=args.svd_method)
     peak = PEAK_FP32_MFMA if args.fp32 else PEAK_FP64_MFMA
     if args.workload == "copy":
         return run_copy(args, eng)

@@ -117,7 +117,8 @@ typedef struct {
     int32_t b;         /* rows per block                                  */
     int32_t zbytes;    /* bytes per working-matrix element: 16 (complex
                           FP64 rotations) or 8 (complex FP32)             */
-    int32_t pad;
+    int32_t method;    /* 0: one-sided Jacobi on the rows of [X | Y];
+                          1: two-sided Jacobi on the Gram H = X X^H       */
 } mpsx_plan;
 
 typedef struct {
@@ -150,7 +151,14 @@ typedef struct {
                            whose last launched sweep still rotated (the
                            device-side status when check_every = 0); the
                            caller zeroes it                                 */
+    int32_t method;     /* MPSX_SVD_AUTO (0): the Gram method for FP64
+                           problems with b = 16, rows otherwise;
+                           MPSX_SVD_ROWS (1): the one-sided method for all */
+    int32_t reserved;
 } mpsx_svd_opts;
+
+#define MPSX_SVD_AUTO 0
+#define MPSX_SVD_ROWS 1
 
 /* Validates the batch, fills every h[i].plan and returns the workspace bytes
  * mpsx_two_site needs. Call it before copying h to the device. */

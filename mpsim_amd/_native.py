@@ -25,7 +25,7 @@ SITE_OP = np.dtype([("a", "<u8"), ("out", "<u8"), ("gate", "<u8"), ("chi_l", "<i
 PLAN_FIELDS = [("ws_off", "<i8"), ("tile_base", "<i8"), ("pair_base", "<i8"),
                ("row_base", "<i8"), ("out_base", "<i8"), ("C", "<i4"), ("R", "<i4"),
                ("Cp", "<i4"), ("LD", "<i4"), ("nb", "<i4"), ("bw", "<i4"), ("zbytes", "<i4"),
-               ("pad", "<i4")]
+               ("method", "<i4")]
 PLAN = np.dtype(PLAN_FIELDS)
 TWO_SITE_OP = np.dtype([("a", "<u8"), ("b", "<u8"), ("gate", "<u8"), ("a_out", "<u8"),
                         ("b_out", "<u8"), ("s_out", "<u8"), ("discarded", "<u8"),
@@ -45,7 +45,8 @@ class SvdOpts(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("max_sweeps", ctypes.c_int32),
                 ("check_every", ctypes.c_int32), ("inner_sweeps", ctypes.c_int32),
                 ("tol", ctypes.c_double), ("block_switch", ctypes.c_int32),
-                ("rotation_fp32", ctypes.c_int32), ("unconverged", ctypes.c_void_p)]
+                ("rotation_fp32", ctypes.c_int32), ("unconverged", ctypes.c_void_p),
+                ("method", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class MpsxError(RuntimeError):

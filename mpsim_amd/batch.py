@@ -91,13 +91,20 @@ def reset(dc: DeviceCircuit) -> None:
     dc.discarded = []
 
 
+def ws_bound(C: int, R: int) -> int:
+    """Upper bound of the Jacobi workspace bytes of one C x R problem:
+    complex FP64 working rows [X | Y] (padded rows Cp <= C + 31), plus the
+    Gram method's Cp x max(Cp, R) Gram and 32x32 pivot rotations."""
+    cp = C + 31
+    return 16 * cp * (R + cp + 4) + 16 * cp * (max(cp, R) + 33) + 1024
+
+
 def _ws_estimate(it: TwoSiteItem) -> int:
     """Upper bound of one problem's Jacobi workspace bytes."""
     cl, d, _ = it.a.shape
     cr = it.b.shape[2]
     C, R = (d * cr, d * cl) if it.absorb_left else (d * cl, d * cr)
-    cp = C + 31
-    return 16 * cp * (R + cp + 4) + 16 * cp + 1024  # complex FP64 working rows
+    return ws_bound(C, R)
 
 
 def tick(engine: Engine, dcs: Sequence[DeviceCircuit], svd: Optional[SvdConfig] = None,

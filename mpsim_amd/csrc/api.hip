@@ -48,6 +48,9 @@ constexpr int kDefaultInnerSweeps = 1;
 constexpr double kDefaultTol64 = 1e-7, kDefaultTol32 = 1e-5;
 constexpr double kNullRel64 = 1e-9, kNullRel32 = 1e-6;
 constexpr int64_t kGridLimit = 2147483647LL;
+// Absolute floor of the Gram method's rotation test, relative to
+// ||theta||_F^2: above the FP64 rounding that H gathers over the rounds.
+constexpr double kGramAbsRel = 1e-12;
 
 struct Resolved {
     int block, max_sweeps, check_every, inner_sweeps;
@@ -57,10 +60,12 @@ struct Resolved {
     bool fp32;
     double null_rel;
     int zbytes;
+    int method;
 };
 
 Resolved resolve(const mpsx_svd_opts *o) {
-    Resolved r{0, kDefaultMaxSweeps, 1, kDefaultInnerSweeps, 0.0, 64, nullptr, false, 0.0, 16};
+    Resolved r{0, kDefaultMaxSweeps, 1, kDefaultInnerSweeps, 0.0, 64, nullptr, false, 0.0, 16,
+               MPSX_SVD_AUTO};
     if (o) {
         if (o->block) r.block = o->block;
         if (o->max_sweeps) r.max_sweeps = o->max_sweeps;
@@ -70,6 +75,7 @@ Resolved resolve(const mpsx_svd_opts *o) {
         if (o->block_switch > 0) r.block_switch = o->block_switch;
         r.unconverged = o->unconverged;
         r.fp32 = o->rotation_fp32 != 0;
+        r.method = o->method;
     }
     if (r.tol <= 0) r.tol = r.fp32 ? kDefaultTol32 : kDefaultTol64;
     r.null_rel = r.fp32 ? kNullRel32 : kNullRel64;
@@ -140,6 +146,12 @@ int64_t theta_tiles(const mpsx_two_site_op &o) {
 // there are more than `switch_rows` rows (the autotuner measures the
 // crossover; 64 by default).
 int auto_block(int C, int switch_rows) { return C <= switch_rows ? 8 : 16; }
+
+// The Gram method serves FP64 problems with 32-row pivots unless the
+// one-sided method is asked for.
+int plan_method(const Resolved &r, int b) {
+    return (!r.fp32 && b == 16 && r.method != MPSX_SVD_ROWS) ? 1 : 0;
+}
 
 int *poll_buffer() {
     thread_local int *buf = nullptr;
@@ -260,6 +272,8 @@ int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts
     if (r.block != 0 && r.block != 8 && r.block != 16)
         return fail(MPSX_EINVAL, "block %d not in {0 (auto), 8, 16}", r.block);
     if (r.max_sweeps < 1 || r.max_sweeps > 64) return fail(MPSX_EINVAL, "max_sweeps outside [1,64]");
+    if (r.method != MPSX_SVD_AUTO && r.method != MPSX_SVD_ROWS)
+        return fail(MPSX_EINVAL, "method %d not in {0 (auto), 1 (rows)}", r.method);
     int64_t off = kHeaderBytes, tiles = 0, pairs = 0, rows = 0, outs = 0;
     for (int i = 0; i < count; ++i) {
         mpsx_two_site_op &o = h[i];
@@ -295,7 +309,7 @@ int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts
         }
         p.LD = (int32_t)align_up((int64_t)p.R + p.Cp, 4);
         p.zbytes = r.zbytes;
-        p.pad = 0;
+        p.method = p.nb > 0 ? plan_method(r, p.b) : 0;
         p.ws_off = off;
         p.tile_base = tiles;
         p.pair_base = pairs;
@@ -331,9 +345,10 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
     // Re-derive totals from the planned host table and check the plan is
     // consistent (a table that was not planned, or edited, is rejected).
     int64_t need = kHeaderBytes, tiles = 0, pairs = 0, rows = 0, outs = 0;
-    int max_nb = 0;
+    int rows_nb = 0, gram_nb = 0, gram_y = 0, gram_c = 0;
     bool has_d[5] = {false, false, false, false, false};  // index 1: plain splits
-    bool has_b[17] = {false};
+    bool has_b[17] = {false};  // one-sided problems by block width
+    bool has_gram = false;
     for (int i = 0; i < count; ++i) {
         const mpsx_two_site_op &o = h[i];
         const mpsx_plan &p = o.plan;
@@ -344,14 +359,24 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
             return fail(MPSX_EINVAL, "op %d: invalid plan", i);
         if (p.zbytes != r.zbytes)
             return fail(MPSX_EINVAL, "op %d: planned for the other rotation precision", i);
+        if (p.method != (p.nb > 0 ? plan_method(r, p.b) : 0))
+            return fail(MPSX_EINVAL, "op %d: planned for another SVD method", i);
         need += problem_bytes(p);
         tiles += theta_tiles(o);
         pairs += p.nb / 2;
         rows += p.Cp;
         outs += gather_units((int64_t)o.d * o.chi_l, o.k);
-        max_nb = std::max(max_nb, p.nb);
         has_d[o.d] = true;
-        if (p.nb > 0) has_b[p.b] = true;
+        if (p.nb > 0 && p.method == 0) {
+            has_b[p.b] = true;
+            rows_nb = std::max(rows_nb, p.nb);
+        }
+        if (p.nb > 0 && p.method == 1) {
+            has_gram = true;
+            gram_nb = std::max(gram_nb, p.nb);
+            gram_y = std::max(gram_y, p.nb / 4 + (p.Cp + 63) / 64);
+            gram_c = std::max(gram_c, (p.R + 31) / 32);
+        }
     }
     const int64_t owner_off = align_up(need, 256);
     need = owner_off + align_up(4 * pairs, 256);
@@ -376,10 +401,12 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
     bool converged = false;
     if (pairs > 0)
         HIP_TRY(timed("k_owner", stream, [&] { return launch_owner(stream, d, count, owner); }), "k_owner");
+    if (has_gram)
+        HIP_TRY(timed("k_hgram", stream, [&] { return launch_hgram(stream, d, owner, ws, pairs, gram_nb); }), "k_hgram");
     if (pairs > 0) {
         int *pin = can_poll ? poll_buffer() : nullptr;
         for (int s = 0; s < r.max_sweeps; ++s) {
-            for (int round = 0; round < max_nb - 1; ++round)
+            for (int round = 0; round < rows_nb - 1; ++round)
                 for (int b : {8, 16})
                     if (has_b[b])
                         HIP_TRY(timed(b == 8 ? (f32 ? "k_jacobi_round32<16>" : "k_jacobi_round64<16>")
@@ -390,6 +417,17 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
                                                                      r.tol, r.tol * 0.01, r.inner_sweeps);
                                       }),
                                 "k_jacobi_round");
+            for (int round = 0; has_gram && round < gram_nb - 1; ++round) {
+                HIP_TRY(timed("k_hsolve", stream,
+                              [&] {
+                                  return launch_hsolve(stream, d, owner, ws, pairs, round, r.tol, r.tol * 0.01,
+                                                       r.inner_sweeps, kGramAbsRel);
+                              }),
+                        "k_hsolve");
+                HIP_TRY(timed("k_hupdate", stream,
+                              [&] { return launch_hupdate(stream, d, owner, ws, pairs, gram_y, round); }),
+                        "k_hupdate");
+            }
             HIP_TRY(timed("k_sweep_end", stream, [&] { return launch_sweep_end(stream, d, count, ws, s); }), "k_sweep_end");
             ++sweeps;
             if (pin && ((s + 1) % r.check_every == 0 || s + 1 == r.max_sweeps)) {
@@ -408,6 +446,10 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
         converged = true;
     }
 
+    if (has_gram) {
+        HIP_TRY(timed("k_hxcopy", stream, [&] { return launch_hxcopy(stream, d, count, ws, rows); }), "k_hxcopy");
+        HIP_TRY(timed("k_hxgemm", stream, [&] { return launch_hxgemm(stream, d, owner, ws, pairs, gram_c); }), "k_hxgemm");
+    }
     if (rows > 0) {
         HIP_TRY(timed("k_rownorm", stream, [&] { return launch_rownorm(stream, f32, d, count, ws, rows); }), "k_rownorm");
         HIP_TRY(timed("k_sort", stream, [&] { return launch_sort(stream, d, count, ws, r.unconverged); }), "k_sort");

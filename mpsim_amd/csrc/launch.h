@@ -30,3 +30,13 @@ hipError_t launch_sort(hipStream_t s, const mpsx_two_site_op *d, int count, char
                        int *unconverged);
 hipError_t launch_gather(hipStream_t s, bool f32, const mpsx_two_site_op *d, int count, char *ws,
                          int64_t outs);
+hipError_t launch_hgram(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+                        int64_t pairs, int max_nb);
+hipError_t launch_hsolve(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+                         int64_t pairs, int round, double tol, double tol_inner, int inner_sweeps,
+                         double abs_rel);
+hipError_t launch_hupdate(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+                          int64_t pairs, int ytiles, int round);
+hipError_t launch_hxcopy(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int64_t rows);
+hipError_t launch_hxgemm(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+                         int64_t pairs, int ctiles);

@@ -308,14 +308,17 @@ MPSX_DEV double unit_cos(double y) {
     return r;
 }
 
-// Convergence test and eigensolve of one pivot (shared by both precisions).
-// H: the pivot's Hermitian Gram (FP64, LDS); Q (LDS, identity on entry)
-// accumulates the rotations. Returns false in every thread when no pair of
-// the pivot exceeds the tolerance (nothing to rotate).
+// Convergence test and eigensolve of one pivot (shared by all round
+// kernels). H: the pivot's Hermitian Gram (FP64, LDS); Q (LDS, identity on
+// entry) accumulates the rotations. A pair rotates when |h_ij|^2 exceeds
+// both tol^2 h_ii h_jj and abs2 (an absolute floor, 0 for Grams formed from
+// the rows; the Gram method, whose H carries FP64 rounding from every
+// round, sets it above that noise; inner steps use 1e-4 abs2). Returns false
+// in every thread when no pair of the pivot exceeds the tolerance.
 template <int B2>
 MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *prm_c,
                           double2 *prm_s, int &s_any, ProblemCtl *ctl, double null2, double tol,
-                          double tol_inner, int inner_sweeps) {
+                          double tol_inner, int inner_sweeps, double abs2) {
     constexpr int b = B2 / 2;
     const int tid = threadIdx.x;
     // ---- convergence test on every off-diagonal pair of the pivot (the
@@ -328,7 +331,7 @@ MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *pr
         const double hi = H[i][i].x, hj = H[j][j].x;
         const double2 h = H[i][j];
         const double g2 = h.x * h.x + h.y * h.y;
-        if (hi > null2 && hj > null2 && g2 > tol * tol * hi * hj) rot = 1;
+        if (hi > null2 && hj > null2 && g2 > tol * tol * hi * hj && g2 > abs2) rot = 1;
     }
     if (!__syncthreads_or(rot)) return false;
     if (tid == 0) atomicAdd(&ctl->counter, 1);
@@ -363,6 +366,7 @@ MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *pr
         qk[t] = ok ? j % b : 0;
     }
     auto nz = [](double2 v) { return v.x != 0.0 || v.y != 0.0; };
+    const double abs2_inner = 1e-4 * abs2;
     // step_any[step & 1]: some pair of the step rotates. The parameter lanes
     // and lane 0, which clears the other entry for the next step, are all in
     // wave 0, so their LDS writes stay in program order.
@@ -383,7 +387,8 @@ MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *pr
                 const double g2 = h.x * h.x + h.y * h.y;
                 double cs = 1.0;
                 double2 s = make_double2(0.0, 0.0);
-                if (a > null2 && c > null2 && g2 > tol_inner * tol_inner * a * c && g2 > 0.0) {
+                if (a > null2 && c > null2 && g2 > tol_inner * tol_inner * a * c && g2 > 0.0 &&
+                    g2 > abs2_inner) {
                     // tan of the rotation angle t = w |h| with
                     // w = sign(tau) / (|tau| + sqrt(tau^2 + |h|^2)), tau = (c - a)/2;
                     // s = sn e^{i phi} = (t cs / |h|) h = w cs h.
@@ -482,7 +487,7 @@ k_jacobi_round32(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
 
     const int pi = owner[blockIdx.x];
     const mpsx_plan p = ops[pi].plan;
-    if (p.b != b) return;
+    if (p.b != b || p.method != 0) return;
     if (round >= p.nb - 1) return;
     ProblemCtl *ctl = ctl_ptr(ws, p);
     if (ctl->conv) return;
@@ -591,7 +596,7 @@ k_jacobi_round32(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
     }
     __syncthreads();
 
-    if (!pivot_solve<B2>(H, Q, prm_c, prm_s, s_any, ctl, null2, tol, tol_inner, inner_sweeps))
+    if (!pivot_solve<B2>(H, Q, prm_c, prm_s, s_any, ctl, null2, tol, tol_inner, inner_sweeps, 0.0))
         return;
 
     // ---- apply on the matrix cores. T[i][j] (i, j < NS) from Q:
@@ -698,7 +703,7 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
 
     const int pi = owner[blockIdx.x];
     const mpsx_plan p = ops[pi].plan;
-    if (p.b != b) return;
+    if (p.b != b || p.method != 0) return;
     if (round >= p.nb - 1) return;
     ProblemCtl *ctl = ctl_ptr(ws, p);
     if (ctl->conv) return;
@@ -827,7 +832,7 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
     }
     __syncthreads();
 
-    if (!pivot_solve<B2>(H, Q, prm_c, prm_s, s_any, ctl, null2, tol, tol_inner, inner_sweeps))
+    if (!pivot_solve<B2>(H, Q, prm_c, prm_s, s_any, ctl, null2, tol, tol_inner, inner_sweeps, 0.0))
         return;
 
     // ---- apply on the matrix cores: Z' = Q^H Z for the pivot rows. Lane
@@ -884,6 +889,335 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
         }
 #pragma unroll
         for (int s = 0; s < B2 / 4; ++s) zin[s] = znx[s];
+    }
+}
+
+// ---------------------------------------------------------------- Gram method
+// Two-sided Jacobi on the Hermitian Gram H = X X^H (FP64), for problems
+// planned with method 1 (FP64, 32-row pivots). It is the one-sided
+// iteration above written on H: a round diagonalises the same pivot Grams
+// with the same rotations J = diag(Q_P) and sets H <- J^H H J and
+// Y <- J^H Y. The one-sided round forms each pivot's Gram from its rows,
+// so it reads the pivot rows of [X | Y] twice and writes them once per
+// round; here H is formed once, a round reads and writes the upper block
+// tiles of H and the rows of Y, and X is rotated once at the end,
+// Xt = Y X. A round is two launches: k_hsolve (one workgroup per pivot:
+// convergence test and eigensolve of the pivot block of H, latency-bound)
+// and k_hupdate (one workgroup per pair of pivots and per 64 columns of Y:
+// matrix-core products, memory-bound), so the eigensolves of one call can
+// overlap the updates of another call running on another stream.
+//
+// H is stored as Cp x Cp (row stride Cp) with the 16x16 block tiles
+// (bi <= bj) valid; of the diagonal block tiles only the upper triangles are
+// kept current. Element (i, j) of the lower triangle is conj(H[j][i]).
+MPSX_DEV double2 *h_ptr(char *ws, const mpsx_plan &p) {
+    return reinterpret_cast<double2 *>(ws + h_off(p));
+}
+MPSX_DEV double2 *qmat_ptr(char *ws, const mpsx_plan &p) {
+    return reinterpret_cast<double2 *>(ws + qmat_off(p));
+}
+MPSX_DEV int *qflag_ptr(char *ws, const mpsx_plan &p) {
+    return reinterpret_cast<int *>(ws + qflag_off(p));
+}
+
+// One workgroup per upper 16x16 block tile (bi <= bj) of H = X X^H. x: the
+// problem's pivot slot j (nb/2 per problem), y in [0, nb]; the tile is
+// number j (nb + 1) + y of the nb (nb + 1) / 2 upper tiles in row-major
+// order. Waves take 16-column blocks w, w + 4, ...; three real products per
+// tile as in the round kernel's Gram; partials summed in a fixed order.
+__global__ void __launch_bounds__(256)
+k_hgram(const mpsx_two_site_op *ops, const int *owner, char *ws) {
+    __shared__ double2 part[4][16][17];
+    const int pi = owner[blockIdx.x];
+    const mpsx_plan p = ops[pi].plan;
+    if (p.method != 1) return;
+    const int nb = p.nb;
+    if ((int)blockIdx.y > nb) return;
+    int rem = (int)(blockIdx.x - p.pair_base) * (nb + 1) + (int)blockIdx.y;
+    int bi = 0;
+    while (rem >= nb - bi) {
+        rem -= nb - bi;
+        ++bi;
+    }
+    const int bj = bi + rem;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const double2 *Z = z_ptr<double>(ws, p);
+    const int LD = p.LD, R = p.R;
+    const int64_t ra = (int64_t)(bi * 16 + l16) * LD, rb = (int64_t)(bj * 16 + l16) * LD;
+    mfma_d4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = p1, p3 = p1;
+    for (int c0 = wave * 16; c0 < R; c0 += 64) {
+        const int cq = c0 + 4 * lq;
+        double2 va[4], vb[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int c = cq + k;
+            va[k] = c < R ? Z[ra + c] : make_double2(0.0, 0.0);
+            vb[k] = c < R ? Z[rb + c] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            p1 = mfma16d(va[k].x, vb[k].x, p1);
+            p2 = mfma16d(va[k].y, vb[k].y, p2);
+            p3 = mfma16d(va[k].x + va[k].y, vb[k].x - vb[k].y, p3);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        part[wave][4 * r + lq][l16] = make_double2(p1[r] + p2[r], p3[r] - p1[r] + p2[r]);
+    __syncthreads();
+    const int i = tid >> 4, j = tid & 15;
+    const double2 a = part[0][i][j], b = part[1][i][j], c = part[2][i][j], d = part[3][i][j];
+    double2 v = make_double2((a.x + c.x) + (b.x + d.x), (a.y + c.y) + (b.y + d.y));
+    if (bi == bj && i == j) v.y = 0.0;
+    h_ptr(ws, p)[(int64_t)(bi * 16 + i) * p.Cp + bj * 16 + j] = v;
+}
+
+// Round, part 1: one workgroup per pivot (blocks I < J of round `round`).
+// Loads the pivot's 32x32 block of H, runs the convergence test and the
+// pivot eigensolve, and writes Q and the rotated block (Q^H H_PP Q, which
+// the eigensolve leaves in LDS) back; flag[P] tells part 2 whether Q is
+// the identity. The absolute floor abs_rel * ||theta||_F^2 sits above the
+// FP64 rounding H gathers over the rounds (pairs below it would otherwise
+// keep rotating on noise; their residual coupling changes the truncated
+// factors only to second order).
+__global__ void __launch_bounds__(256)
+k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
+         double tol_inner, int inner_sweeps, double abs_rel) {
+    constexpr int B2 = 32, b = 16;
+    __shared__ double2 H[B2][B2 + 1];
+    __shared__ double2 Q[B2][B2 + 1];
+    __shared__ double prm_c[b];
+    __shared__ double2 prm_s[b];
+    __shared__ int s_any;
+    const int pi = owner[blockIdx.x];
+    const mpsx_plan p = ops[pi].plan;
+    if (p.method != 1 || round >= p.nb - 1) return;
+    ProblemCtl *ctl = ctl_ptr(ws, p);
+    if (ctl->conv) return;
+    const int pair = (int)(blockIdx.x - p.pair_base);
+    int I, J;
+    rr_pair(pair, round, p.nb, &I, &J);
+    auto grow = [&](int i) { return i < b ? I * b + i : J * b + (i - b); };
+    double2 *Hg = h_ptr(ws, p);
+    const int Cp = p.Cp, tid = threadIdx.x;
+    // element (u, v), u <= v, lies in a stored tile: (I, I), (I, J) or (J, J)
+    for (int e = tid; e < B2 * B2; e += 256) {
+        const int i = e / B2, j = e % B2;
+        const int u = i <= j ? i : j, v = i <= j ? j : i;
+        double2 h = Hg[(int64_t)grow(u) * Cp + grow(v)];
+        if (i > j) h.y = -h.y;
+        if (i == j) h.y = 0.0;
+        H[i][j] = h;
+        Q[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+    const double abs_t = abs_rel * ctl->fro2;
+    int *flag = qflag_ptr(ws, p);
+    if (!pivot_solve<B2>(H, Q, prm_c, prm_s, s_any, ctl, ctl->null2, tol, tol_inner, inner_sweeps,
+                         abs_t * abs_t)) {
+        if (tid == 0) flag[pair] = 0;
+        return;
+    }
+    double2 *Qg = qmat_ptr(ws, p) + (int64_t)pair * B2 * B2;
+    for (int e = tid; e < B2 * B2; e += 256) {
+        const int i = e / B2, j = e % B2;
+        Qg[e] = Q[i][j];
+        if (i <= j) Hg[(int64_t)grow(i) * Cp + grow(j)] = H[i][j];
+    }
+    if (tid == 0) flag[pair] = 1;
+}
+
+// Round, part 2. x: pivot P of a problem; y < n/2 (n = nb/2 pivots): the
+// 32x32 tile of H between P and P2 = (P + y + 1) mod n, each unordered pair
+// once, T <- Q_P^H T Q_P2 (a factor whose pivot did not rotate is the
+// identity and skipped); y >= n/2: columns 64 (y - n/2) .. +63 of the Y
+// rows of P, Y <- Q_P^H Y. FP64 16x16x4 MFMAs, three real products per
+// complex product; wave w owns output tile (w / 2, w % 2) of the H tile, or
+// 16 columns of all 32 Y rows.
+__global__ void __launch_bounds__(256)
+k_hupdate(const mpsx_two_site_op *ops, const int *owner, char *ws, int round) {
+    constexpr int b = 16;
+    __shared__ double2 Ws[32][33];
+    const int pi = owner[blockIdx.x];
+    const mpsx_plan p = ops[pi].plan;
+    if (p.method != 1 || round >= p.nb - 1) return;
+    if (ctl_ptr(ws, p)->conv) return;
+    const int n = p.nb / 2, nt = n / 2;
+    const int P = (int)(blockIdx.x - p.pair_base), y = (int)blockIdx.y;
+    const int *flag = qflag_ptr(ws, p);
+    const double2 *Qall = qmat_ptr(ws, p);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    int I, J;
+    rr_pair(P, round, p.nb, &I, &J);
+    if (y < nt) {
+        const int dd = y + 1;
+        if (2 * dd == n && P >= nt) return;
+        const int P2 = (P + dd) % n;
+        const int fa = flag[P], fb = flag[P2];
+        if (!fa && !fb) return;
+        int I2, J2;
+        rr_pair(P2, round, p.nb, &I2, &J2);
+        double2 *Hg = h_ptr(ws, p);
+        const int Cp = p.Cp;
+        // tile element (i, j): H row of pivot P's row i, column of pivot P2's
+        // row j; stored in the upper block tile, else as its conjugate
+        auto hload = [&](int i, int j) {
+            const int gi = i < b ? I * b + i : J * b + (i - b);
+            const int gj = j < b ? I2 * b + j : J2 * b + (j - b);
+            if (gi / b < gj / b) return Hg[(int64_t)gi * Cp + gj];
+            const double2 t = Hg[(int64_t)gj * Cp + gi];
+            return make_double2(t.x, -t.y);
+        };
+        auto hstore = [&](int i, int j, double2 v) {
+            const int gi = i < b ? I * b + i : J * b + (i - b);
+            const int gj = j < b ? I2 * b + j : J2 * b + (j - b);
+            if (gi / b < gj / b)
+                Hg[(int64_t)gi * Cp + gj] = v;
+            else
+                Hg[(int64_t)gj * Cp + gi] = make_double2(v.x, -v.y);
+        };
+        const int tm = wave >> 1, tn = wave & 1;
+        // W = Q_P^H T: register r of lane (l16, lq) is W[16 tm + 4 r + lq][16 tn + l16]
+        double2 w[4];
+        if (fa) {
+            const double2 *Qa = Qall + (int64_t)P * 1024;
+            mfma_d4 a1 = {0.0, 0.0, 0.0, 0.0}, a2 = a1, a3 = a1;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int k = 4 * s + lq;
+                const double2 q = Qa[k * 32 + 16 * tm + l16];  // conj: A = Q^H[16 tm + l16][k]
+                const double2 t = hload(k, 16 * tn + l16);     // B = T[k][16 tn + l16]
+                a1 = mfma16d(q.x, t.x, a1);
+                a2 = mfma16d(q.y, t.y, a2);
+                a3 = mfma16d(q.x - q.y, t.x + t.y, a3);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) w[r] = make_double2(a1[r] + a2[r], a3[r] - a1[r] + a2[r]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) w[r] = hload(16 * tm + 4 * r + lq, 16 * tn + l16);
+        }
+        if (!fb) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) hstore(16 * tm + 4 * r + lq, 16 * tn + l16, w[r]);
+            return;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Ws[16 * tm + 4 * r + lq][16 * tn + l16] = w[r];
+        __syncthreads();
+        const double2 *Qb = Qall + (int64_t)P2 * 1024;
+        mfma_d4 c1 = {0.0, 0.0, 0.0, 0.0}, c2 = c1, c3 = c1;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int k = 4 * s + lq;
+            const double2 wa = Ws[16 * tm + l16][k];           // A = W[16 tm + l16][k]
+            const double2 qb = Qb[k * 32 + 16 * tn + l16];     // B = Q_P2[k][16 tn + l16]
+            c1 = mfma16d(wa.x, qb.x, c1);
+            c2 = mfma16d(wa.y, qb.y, c2);
+            c3 = mfma16d(wa.x + wa.y, qb.x + qb.y, c3);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            hstore(16 * tm + 4 * r + lq, 16 * tn + l16,
+                   make_double2(c1[r] - c2[r], c3[r] - c1[r] - c2[r]));
+        return;
+    }
+    // Y rows of pivot P, 16 columns per wave
+    const int c0 = (y - nt) * 64;
+    if (c0 >= p.Cp || !flag[P]) return;
+    const double2 *Qa = Qall + (int64_t)P * 1024;
+    double2 *Z = z_ptr<double>(ws, p);
+    const int LD = p.LD, R = p.R;
+    const int c = c0 + 16 * wave + l16;
+    const bool ok = c < p.Cp;
+    int64_t roff[8];
+    double2 v[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int i = 4 * s + lq;
+        roff[s] = (int64_t)(i < b ? I * b + i : J * b + (i - b)) * LD + R;
+        v[s] = ok ? Z[roff[s] + c] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int tm = 0; tm < 2; ++tm) {
+        mfma_d4 a1 = {0.0, 0.0, 0.0, 0.0}, a2 = a1, a3 = a1;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const double2 q = Qa[(4 * s + lq) * 32 + 16 * tm + l16];  // conj: A = Q^H[16 tm + l16][k]
+            a1 = mfma16d(q.x, v[s].x, a1);
+            a2 = mfma16d(q.y, v[s].y, a2);
+            a3 = mfma16d(q.x - q.y, v[s].x + v[s].y, a3);
+        }
+        if (ok) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * tm + 4 * r + lq;
+                const int64_t ro = (int64_t)(i < b ? I * b + i : J * b + (i - b)) * LD + R;
+                Z[ro + c] = make_double2(a1[r] + a2[r], a3[r] - a1[r] + a2[r]);
+            }
+        }
+    }
+}
+
+// End of the Gram method, 1: copy X (Cp x R) into the H region (row stride
+// R), one workgroup per padded row.
+__global__ void __launch_bounds__(256)
+k_hxcopy(const mpsx_two_site_op *ops, int count, char *ws) {
+    const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
+                               OP_FIELD(plan.row_base), blockIdx.x);
+    const mpsx_plan p = ops[pi].plan;
+    if (p.method != 1) return;
+    const int row = (int)(blockIdx.x - p.row_base);
+    if (row >= p.Cp) return;
+    const double2 *z = z_ptr<double>(ws, p) + (int64_t)row * p.LD;
+    double2 *x = h_ptr(ws, p) + (int64_t)row * p.R;
+    for (int c = threadIdx.x; c < p.R; c += 256) x[c] = z[c];
+}
+
+// End of the Gram method, 2: X <- Y Xc (Xc the copy), so that the rows of X
+// are those the one-sided iteration would have produced. x: pivot slot j
+// = rows 32 j .. 32 j + 31 (Cp = 32 nb/2), y: 32 columns; wave w computes
+// the 16x16 output tile (w / 2, w % 2) over K = Cp with three real products.
+__global__ void __launch_bounds__(256)
+k_hxgemm(const mpsx_two_site_op *ops, const int *owner, char *ws) {
+    const int pi = owner[blockIdx.x];
+    const mpsx_plan p = ops[pi].plan;
+    if (p.method != 1) return;
+    const int R = p.R, LD = p.LD, Cp = p.Cp;
+    const int col0 = (int)blockIdx.y * 32;
+    if (col0 >= R) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const int tm = wave >> 1, tn = wave & 1;
+    const int row0 = (int)(blockIdx.x - p.pair_base) * 32 + 16 * tm;
+    const int col = col0 + 16 * tn + l16;
+    const bool ok = col < R;
+    double2 *Z = z_ptr<double>(ws, p);
+    const double2 *Xc = h_ptr(ws, p);
+    const double2 *yrow = Z + (int64_t)(row0 + l16) * LD + R;  // A = Y[row0 + l16][k]
+    mfma_d4 a1 = {0.0, 0.0, 0.0, 0.0}, a2 = a1, a3 = a1;
+    for (int k0 = 0; k0 < Cp; k0 += 16) {
+        double2 ya[4], xb[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int k = k0 + 4 * s + lq;
+            ya[s] = yrow[k];
+            xb[s] = ok ? Xc[(int64_t)k * R + col] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            a1 = mfma16d(ya[s].x, xb[s].x, a1);
+            a2 = mfma16d(ya[s].y, xb[s].y, a2);
+            a3 = mfma16d(ya[s].x + ya[s].y, xb[s].x + xb[s].y, a3);
+        }
+    }
+    if (ok) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            Z[(int64_t)(row0 + 4 * r + lq) * LD + col] =
+                make_double2(a1[r] - a2[r], a3[r] - a1[r] - a2[r]);
     }
 }
 
@@ -1101,5 +1435,37 @@ hipError_t launch_gather(hipStream_t s, bool f32, const mpsx_two_site_op *d, int
         hipLaunchKernelGGL(k_gather<float>, dim3((unsigned)outs), dim3(256), 0, s, d, count, ws);
     else
         hipLaunchKernelGGL(k_gather<double>, dim3((unsigned)outs), dim3(256), 0, s, d, count, ws);
+    return hipGetLastError();
+}
+
+hipError_t launch_hgram(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+                        int64_t pairs, int max_nb) {
+    hipLaunchKernelGGL(k_hgram, dim3((unsigned)pairs, (unsigned)(max_nb + 1)), dim3(256), 0, s, d, owner, ws);
+    return hipGetLastError();
+}
+
+hipError_t launch_hsolve(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+                         int64_t pairs, int round, double tol, double tol_inner, int inner_sweeps,
+                         double abs_rel) {
+    hipLaunchKernelGGL(k_hsolve, dim3((unsigned)pairs), dim3(256), 0, s, d, owner, ws, round, tol,
+                       tol_inner, inner_sweeps, abs_rel);
+    return hipGetLastError();
+}
+
+hipError_t launch_hupdate(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+                          int64_t pairs, int ytiles, int round) {
+    hipLaunchKernelGGL(k_hupdate, dim3((unsigned)pairs, (unsigned)ytiles), dim3(256), 0, s, d, owner,
+                       ws, round);
+    return hipGetLastError();
+}
+
+hipError_t launch_hxcopy(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int64_t rows) {
+    hipLaunchKernelGGL(k_hxcopy, dim3((unsigned)rows), dim3(256), 0, s, d, count, ws);
+    return hipGetLastError();
+}
+
+hipError_t launch_hxgemm(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+                         int64_t pairs, int ctiles) {
+    hipLaunchKernelGGL(k_hxgemm, dim3((unsigned)pairs, (unsigned)ctiles), dim3(256), 0, s, d, owner, ws);
     return hipGetLastError();
 }

@@ -66,11 +66,12 @@ class SvdConfig:
     block_switch: int = 0   # 0 = library default (64 rows)
     unconverged: int = 0    # device address of an int32 non-convergence counter, or 0
     fp32: bool = False      # FP32 working matrix (faster, ~1e-6 relative error per update)
+    method: int = 0         # 0: Gram method for FP64 32-row pivots; 1: one-sided everywhere
 
     def to_c(self) -> nat.SvdOpts:
         return nat.SvdOpts(self.block, self.max_sweeps, self.check_every, self.inner_sweeps,
                            self.tol, self.block_switch, 1 if self.fp32 else 0,
-                           self.unconverged or None)
+                           self.unconverged or None, self.method, 0)
 
 
 @dataclass
@@ -121,8 +122,9 @@ class Engine:
         self.svd = tuned_config()
         self.last_sweeps = 0
         self.split_min = 8          # problems with >= SPLIT_ROWS rows needed to split a call
-        self._ws2 = None
-        self._helper = None         # (stream, single-thread executor) for split calls
+        self.split_ways = 2         # concurrent groups of a split call
+        self._wss: list = []        # workspaces of the helper streams' groups
+        self._helpers: list = []    # (stream, single-thread executor) per helper group
         self._keep: list = []       # device tables of the last call, alive until the next
 
     @classmethod
@@ -141,15 +143,17 @@ class Engine:
     def stream(self) -> int:
         return nat.stream_handle(self.dev)
 
-    def workspace(self, nbytes: int, second: bool = False):
-        """Grow-only device workspace (a second one serves the helper stream of
-        split calls)."""
-        cur = self._ws2 if second else self._ws
+    def workspace(self, nbytes: int, slot: int = 0):
+        """Grow-only device workspace; slot g > 0 serves the g-th helper
+        stream of split calls."""
+        while len(self._wss) < slot:
+            self._wss.append(None)
+        cur = self._wss[slot - 1] if slot else self._ws
         if cur is None or cur.numel() < nbytes:
             grow = max(nbytes, 0 if cur is None else int(cur.numel() * 1.25))
             cur = self.torch.empty(max(grow, 1024), dtype=self.torch.uint8, device=self.dev)
-            if second:
-                self._ws2 = cur
+            if slot:
+                self._wss[slot - 1] = cur
             else:
                 self._ws = cur
         return cur
@@ -192,7 +196,7 @@ class Engine:
         del dtab
 
     # ------------------------------------------------------------ two-site
-    def _prepare_two_site(self, items: Sequence[TwoSiteItem], cfg, second: bool = False):
+    def _prepare_two_site(self, items: Sequence[TwoSiteItem], cfg, slot: int = 0):
         """Outputs, planned host table, device table and workspace of one
         library call (allocations on the calling thread's current stream)."""
         torch = self.torch
@@ -247,7 +251,7 @@ class Engine:
         nbytes = ctypes.c_size_t(0)
         nat.check(self.lib.mpsx_two_site_plan(tab.ctypes.data, n, ctypes.byref(cfg),
                                               ctypes.byref(nbytes)))
-        ws = self.workspace(nbytes.value, second)
+        ws = self.workspace(nbytes.value, slot)
         dtab = nat.to_device_table(tab, self.dev)
         return outs, tab, dtab, ws
 
@@ -268,61 +272,70 @@ class Engine:
     def two_site(self, items: Sequence[TwoSiteItem], svd: Optional[SvdConfig] = None
                  ) -> List[TwoSiteResult]:
         """Batched two-site updates on the current stream. A call with at
-        least `split_min` problems of SPLIT_ROWS or more rows runs as two
-        halves of equal work concurrently, the second from a worker thread on
-        a helper stream: the latency-bound pivot eigensolves of one half then
-        overlap the matrix-core phases of the other. Each problem's arithmetic
-        is the same either way."""
+        least `split_min` problems of SPLIT_ROWS or more rows runs as
+        `split_ways` groups of equal work concurrently, group 0 on the current
+        stream and each other group from its own worker thread on its own
+        helper stream: the latency-bound pivot eigensolves of one group then
+        overlap the matrix-core phases of the others. Each problem's
+        arithmetic is the same either way."""
         n = len(items)
         if n == 0:
             return []
         cfg = (svd or self.svd).to_c()
         self._keep = []
         big = [i for i, it in enumerate(items) if _rows(it) >= SPLIT_ROWS]
-        if len(big) < self.split_min or cfg.check_every == 0:
+        ways = max(1, min(int(self.split_ways), n))
+        if ways < 2 or len(big) < self.split_min or cfg.check_every == 0:
             outs, tab, dtab, ws = self._prepare_two_site(items, cfg)
             self._keep = [dtab]
             self.last_sweeps = self._run_two_site(tab, dtab, ws, cfg, self.stream)
             return outs
-        # two groups of equal estimated work (rows^2 * row length per problem)
+        # groups of equal estimated work (rows^2 * row length per problem),
+        # largest problems first, each to the least loaded group
         order = sorted(range(n), key=lambda i: -_cost(items[i]))
-        grp, load = ([], []), [0.0, 0.0]
+        grp = [[] for _ in range(ways)]
+        load = [0.0] * ways
         for i in order:
-            g = 0 if load[0] <= load[1] else 1
+            g = min(range(ways), key=lambda x: load[x])
             grp[g].append(i)
             load[g] += _cost(items[i])
-        ia, ib = sorted(grp[0]), sorted(grp[1])
-        outs_a, tab_a, dtab_a, ws_a = self._prepare_two_site([items[i] for i in ia], cfg)
-        cfg_b = (svd or self.svd).to_c()
-        outs_b, tab_b, dtab_b, ws_b = self._prepare_two_site([items[i] for i in ib], cfg_b, True)
-        self._keep = [dtab_a, dtab_b]
+        grp = [sorted(g) for g in grp if g]
+        preps = []
+        for slot, g in enumerate(grp):
+            c = cfg if slot == 0 else (svd or self.svd).to_c()
+            preps.append((c,) + self._prepare_two_site([items[i] for i in g], c, slot))
+        self._keep = [p[3] for p in preps]
         torch = self.torch
-        if self._helper is None:
-            from concurrent.futures import ThreadPoolExecutor
-            self._helper = (torch.cuda.Stream(self.dev), ThreadPoolExecutor(max_workers=1))
-        hs, pool = self._helper
+        from concurrent.futures import ThreadPoolExecutor
+        while len(self._helpers) < len(grp) - 1:
+            self._helpers.append((torch.cuda.Stream(self.dev), ThreadPoolExecutor(max_workers=1)))
         cur = torch.cuda.current_stream(self.dev)
         fork = torch.cuda.Event()
         fork.record(cur)
-        hs.wait_event(fork)  # inputs and outputs are ready on the helper stream
         dev = self.dev
+        futs = []
+        for slot in range(1, len(grp)):
+            hs, pool = self._helpers[slot - 1]
+            hs.wait_event(fork)  # inputs and outputs are ready on the helper stream
+            c, outs_g, tab, dtab, ws = preps[slot]
 
-        def run_b():
-            torch.cuda.set_device(dev)
-            return self._run_two_site(tab_b, dtab_b, ws_b, cfg_b, int(hs.cuda_stream))
+            def run(c=c, tab=tab, dtab=dtab, ws=ws, hs=hs):
+                torch.cuda.set_device(dev)
+                return self._run_two_site(tab, dtab, ws, c, int(hs.cuda_stream))
 
-        fut = pool.submit(run_b)
-        sw_a = self._run_two_site(tab_a, dtab_a, ws_a, cfg, int(cur.cuda_stream))
-        sw_b = fut.result()
-        join = torch.cuda.Event()
-        join.record(hs)
-        cur.wait_event(join)
-        self.last_sweeps = max(sw_a, sw_b)
+            futs.append(pool.submit(run))
+        c, _, tab, dtab, ws = preps[0]
+        sweeps = [self._run_two_site(tab, dtab, ws, c, int(cur.cuda_stream))]
+        sweeps += [f.result() for f in futs]
+        for slot in range(1, len(grp)):
+            join = torch.cuda.Event()
+            join.record(self._helpers[slot - 1][0])
+            cur.wait_event(join)
+        self.last_sweeps = max(sweeps)
         outs: List[TwoSiteResult] = [None] * n
-        for j, i in enumerate(ia):
-            outs[i] = outs_a[j]
-        for j, i in enumerate(ib):
-            outs[i] = outs_b[j]
+        for g, p in zip(grp, preps):
+            for j, i in enumerate(g):
+                outs[i] = p[1][j]
         return outs
 
     # ------------------------------------------------------------ gemm

@@ -22,14 +22,14 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-from .batch import DeviceCircuit, prepare, tick
+from .batch import DeviceCircuit, prepare, tick, ws_bound
 from .circuits import Circuit, Request, bond_plan
 from .engine import Engine, SvdConfig
 
 
 def workspace_bytes(circuit: Circuit) -> int:
     """Upper bound of the Jacobi workspace of the circuit's largest two-site
-    layer (padded rows Cp <= C + 31, row length R + Cp, complex FP64)."""
+    layer (batch.ws_bound per update)."""
     d = circuit.d
     best = 0
     prev = [1] * (circuit.n + 1)
@@ -39,8 +39,7 @@ def workspace_bytes(circuit: Circuit) -> int:
             if len(q) == 2:
                 i = min(q)
                 M, N = d * prev[i], d * prev[i + 2]
-                cp = M + 31
-                tot += 16 * cp * (N + cp + 4) + 16 * cp + 1024  # complex FP64 rows
+                tot += ws_bound(M, N)
         best = max(best, tot)
         prev = [1] + list(bonds) + [1]
     return best
