@@ -315,17 +315,17 @@ MPSX_DEV double unit_cos(double y) {
 // the rows; the Gram method, whose H carries FP64 rounding from every
 // round, sets it above that noise; inner steps use 1e-4 abs2). Returns false
 // in every thread when no pair of the pivot exceeds the tolerance.
-template <int B2>
+template <int B2, int NT = 256>
 MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *prm_c,
                           double2 *prm_s, int &s_any, ProblemCtl *ctl, double null2, double tol,
                           double tol_inner, int inner_sweeps, double abs2) {
-    constexpr int b = B2 / 2;
+    constexpr int b = B2 / 2, NW = NT / 64;  // NT threads, NW waves
     const int tid = threadIdx.x;
     // ---- convergence test on every off-diagonal pair of the pivot (the
     // blocks' own rows included: a block is only orthogonal internally once
     // it has been part of a rotated pivot)
     int rot = 0;
-    for (int e = tid; e < B2 * B2; e += 256) {
+    for (int e = tid; e < B2 * B2; e += NT) {
         const int i = e / B2, j = e % B2;
         if (i >= j) continue;
         const double hi = H[i][i].x, hj = H[j][j].x;
@@ -337,25 +337,26 @@ MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *pr
     if (tid == 0) atomicAdd(&ctl->counter, 1);
 
     // ---- pivot eigensolve: cyclic Jacobi on H (FP64), Q accumulates.
-    // Work items of a step: the NH = b(b+1)/2 upper 2x2 blocks of H and the
-    // NQ = B2*b (row, pair) entries of Q. They are dealt out per wave so that
-    // lanes of one wave run the same kind of item together (divergent lanes
-    // would serialise the two paths): wave w takes H blocks [w*HPW, (w+1)*HPW)
-    // in one slot (lane < HPW) and Q entries [w*NQ/4, (w+1)*NQ/4) in QS full
-    // slots. Items are fixed across steps; the index pairs of a step are
-    // computed arithmetically. An item whose rotations are all the identity
-    // (s == 0 exactly) is skipped.
-    constexpr int NH = b * (b + 1) / 2, NQ = B2 * b;
-    constexpr int HPW = (NH + 3) / 4, QPW = NQ / 4, NQI = (QPW + 63) / 64;
-    static_assert(HPW <= 64 && NQ % 4 == 0, "pivot too large for one workgroup");
-    const int lane64 = tid & 63, wv = tid >> 6;
-    int hk1 = -1, hk2 = -1;
-    if (lane64 < HPW && wv * HPW + lane64 < NH) {
-        int k1 = 0, rem = wv * HPW + lane64;
-        while (rem >= b - k1) { rem -= b - k1; ++k1; }
-        hk1 = k1;
-        hk2 = k1 + rem;
+    // The index pairs of every step come from a table built once (pt[step][k]
+    // = (p, q) of pair k), so a step carries no index arithmetic. Work items
+    // of a step: the b*b 2x2 blocks (k1, k2) of H, B <- J_k1^H B J_k2, taken
+    // by threads 0 .. b*b-1 (every entry of H lies in exactly one block, so
+    // a step reads and writes disjoint entries and needs no mirror), and the
+    // NQ = B2*b (row, pair) entries of Q, Q <- Q J, dealt out per wave in
+    // NQI full slots. An item whose rotations are all the identity (s == 0
+    // exactly) is skipped.
+    constexpr int NB = b * b, NQ = B2 * b;
+    constexpr int QPW = NQ / NW, NQI = (QPW + 63) / 64;
+    static_assert(NB <= NT && NQ % NW == 0, "pivot too large for one workgroup");
+    __shared__ unsigned char pt[B2 - 1][b][2];
+    for (int e = tid; e < (B2 - 1) * b; e += NT) {
+        int pp, qq;
+        inner_pair(e % b, e / b, b, &pp, &qq);
+        pt[e / b][e % b][0] = (unsigned char)pp;
+        pt[e / b][e % b][1] = (unsigned char)qq;
     }
+    const int lane64 = tid & 63, wv = tid >> 6;
+    const int hk1 = tid < NB ? tid / b : -1, hk2 = tid % b;
     int qrow[NQI], qk[NQI];
 #pragma unroll
     for (int t = 0; t < NQI; ++t) {
@@ -372,7 +373,7 @@ MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *pr
     // wave 0, so their LDS writes stay in program order.
     __shared__ int step_any[2];
     for (int sw = 0; sw < inner_sweeps; ++sw) {
-        __syncthreads();  // every wave has read s_any of the previous sweep
+        __syncthreads();  // every wave has read s_any of the previous sweep (and pt is built)
         if (tid == 0) {
             s_any = 0;
             step_any[0] = 0;
@@ -380,8 +381,7 @@ MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *pr
         __syncthreads();
         for (int step = 0; step < B2 - 1; ++step) {
             if (tid < b) {
-                int pp, qq;
-                inner_pair(tid, step, b, &pp, &qq);
+                const int pp = pt[step][tid][0], qq = pt[step][tid][1];
                 const double a = H[pp][pp].x, c = H[qq][qq].x;
                 const double2 h = H[pp][qq];
                 const double g2 = h.x * h.x + h.y * h.y;
@@ -407,15 +407,14 @@ MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *pr
             const bool any = step_any[step & 1] != 0;
             if (tid == 0) step_any[(step + 1) & 1] = 0;
             if (!any) continue;  // nothing to rotate in this step: H and Q unchanged
-            // H <- J^H H J on 2x2 blocks (upper blocks, mirrored), Q <- Q J.
+            // H <- J^H H J on the 2x2 block (k1, k2), Q <- Q J, with
             // J = [[c, s], [-conj(s), c]] for each index pair (p, q).
             if (hk1 >= 0) {
                 const int k1 = hk1, k2 = hk2;
                 const double2 s1 = prm_s[k1], s2 = prm_s[k2];
                 if (nz(s1) || nz(s2)) {
-                    int p1, q1, p2, q2;
-                    inner_pair(k1, step, b, &p1, &q1);
-                    inner_pair(k2, step, b, &p2, &q2);
+                    const int p1 = pt[step][k1][0], q1 = pt[step][k1][1];
+                    const int p2 = pt[step][k2][0], q2 = pt[step][k2][1];
                     const double c1 = prm_c[k1], c2 = prm_c[k2];
                     const double2 h00 = H[p1][p2], h01 = H[p1][q2], h10 = H[q1][p2], h11 = H[q1][q2];
                     const double2 ms2 = make_double2(-s2.x, s2.y);  // -conj(s2)
@@ -425,20 +424,10 @@ MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *pr
                     const double2 m11 = zadd(zmul(h10, s2), zscale(h11, c2));
                     const double2 ns1 = make_double2(-s1.x, -s1.y);  // -s1
                     const double2 cs1 = zconj(s1);
-                    const double2 n00 = zadd(zscale(m00, c1), zmul(ns1, m10));
-                    const double2 n01 = zadd(zscale(m01, c1), zmul(ns1, m11));
-                    const double2 n10 = zadd(zmul(cs1, m00), zscale(m10, c1));
-                    const double2 n11 = zadd(zmul(cs1, m01), zscale(m11, c1));
-                    H[p1][p2] = n00;
-                    H[p1][q2] = n01;
-                    H[q1][p2] = n10;
-                    H[q1][q2] = n11;
-                    if (k1 != k2) {
-                        H[p2][p1] = zconj(n00);
-                        H[q2][p1] = zconj(n01);
-                        H[p2][q1] = zconj(n10);
-                        H[q2][q1] = zconj(n11);
-                    }
+                    H[p1][p2] = zadd(zscale(m00, c1), zmul(ns1, m10));
+                    H[p1][q2] = zadd(zscale(m01, c1), zmul(ns1, m11));
+                    H[q1][p2] = zadd(zmul(cs1, m00), zscale(m10, c1));
+                    H[q1][q2] = zadd(zmul(cs1, m01), zscale(m11, c1));
                 }
             }
 #pragma unroll
@@ -447,8 +436,7 @@ MPSX_DEV bool pivot_solve(double2 (*H)[B2 + 1], double2 (*Q)[B2 + 1], double *pr
                 const int row = qrow[t], k = qk[t];
                 const double2 s = prm_s[k];
                 if (!nz(s)) continue;
-                int pp, qq;
-                inner_pair(k, step, b, &pp, &qq);
+                const int pp = pt[step][k][0], qq = pt[step][k][1];
                 const double c = prm_c[k];
                 const double2 qp = Q[row][pp], qv = Q[row][qq];
                 const double2 ms = make_double2(-s.x, s.y);  // -conj(s)
@@ -973,6 +961,10 @@ k_hgram(const mpsx_two_site_op *ops, const int *owner, char *ws) {
     h_ptr(ws, p)[(int64_t)(bi * 16 + i) * p.Cp + bj * 16 + j] = v;
 }
 
+// Workgroup size of the eigensolve kernel (eight waves measured slower than
+// four: the per-step cost is latency, not issue).
+constexpr int HSOLVE_THREADS = 256;
+
 // Round, part 1: one workgroup per pivot (blocks I < J of round `round`).
 // Loads the pivot's 32x32 block of H, runs the convergence test and the
 // pivot eigensolve, and writes Q and the rotated block (Q^H H_PP Q, which
@@ -981,10 +973,10 @@ k_hgram(const mpsx_two_site_op *ops, const int *owner, char *ws) {
 // FP64 rounding H gathers over the rounds (pairs below it would otherwise
 // keep rotating on noise; their residual coupling changes the truncated
 // factors only to second order).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(HSOLVE_THREADS)
 k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
          double tol_inner, int inner_sweeps, double abs_rel) {
-    constexpr int B2 = 32, b = 16;
+    constexpr int B2 = 32, b = 16, NT = HSOLVE_THREADS;
     __shared__ double2 H[B2][B2 + 1];
     __shared__ double2 Q[B2][B2 + 1];
     __shared__ double prm_c[b];
@@ -1001,26 +993,27 @@ k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, dou
     auto grow = [&](int i) { return i < b ? I * b + i : J * b + (i - b); };
     double2 *Hg = h_ptr(ws, p);
     const int Cp = p.Cp, tid = threadIdx.x;
-    // element (u, v), u <= v, lies in a stored tile: (I, I), (I, J) or (J, J)
-    for (int e = tid; e < B2 * B2; e += 256) {
+    // element (i, j), i <= j, lies in a stored tile: (I, I), (I, J) or (J, J);
+    // read row-contiguously, mirrored into the lower triangle
+    for (int e = tid; e < B2 * B2; e += NT) {
         const int i = e / B2, j = e % B2;
-        const int u = i <= j ? i : j, v = i <= j ? j : i;
-        double2 h = Hg[(int64_t)grow(u) * Cp + grow(v)];
-        if (i > j) h.y = -h.y;
+        Q[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        if (i > j) continue;
+        double2 h = Hg[(int64_t)grow(i) * Cp + grow(j)];
         if (i == j) h.y = 0.0;
         H[i][j] = h;
-        Q[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        H[j][i] = make_double2(h.x, -h.y);
     }
     __syncthreads();
     const double abs_t = abs_rel * ctl->fro2;
     int *flag = qflag_ptr(ws, p);
-    if (!pivot_solve<B2>(H, Q, prm_c, prm_s, s_any, ctl, ctl->null2, tol, tol_inner, inner_sweeps,
+    if (!pivot_solve<B2, NT>(H, Q, prm_c, prm_s, s_any, ctl, ctl->null2, tol, tol_inner, inner_sweeps,
                          abs_t * abs_t)) {
         if (tid == 0) flag[pair] = 0;
         return;
     }
     double2 *Qg = qmat_ptr(ws, p) + (int64_t)pair * B2 * B2;
-    for (int e = tid; e < B2 * B2; e += 256) {
+    for (int e = tid; e < B2 * B2; e += NT) {
         const int i = e / B2, j = e % B2;
         Qg[e] = Q[i][j];
         if (i <= j) Hg[(int64_t)grow(i) * Cp + grow(j)] = H[i][j];
@@ -1034,10 +1027,12 @@ k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, dou
 // identity and skipped); y >= n/2: columns 64 (y - n/2) .. +63 of the Y
 // rows of P, Y <- Q_P^H Y. FP64 16x16x4 MFMAs, three real products per
 // complex product; wave w owns output tile (w / 2, w % 2) of the H tile, or
-// 16 columns of all 32 Y rows.
+// 16 columns of all 32 Y rows. Tiles pass through LDS (Ts: T, then the
+// result; Ws: W = Q_P^H T).
 __global__ void __launch_bounds__(256)
 k_hupdate(const mpsx_two_site_op *ops, const int *owner, char *ws, int round) {
     constexpr int b = 16;
+    __shared__ double2 Ts[32][33];
     __shared__ double2 Ws[32][33];
     const int pi = owner[blockIdx.x];
     const mpsx_plan p = ops[pi].plan;
@@ -1061,26 +1056,32 @@ k_hupdate(const mpsx_two_site_op *ops, const int *owner, char *ws, int round) {
         rr_pair(P2, round, p.nb, &I2, &J2);
         double2 *Hg = h_ptr(ws, p);
         const int Cp = p.Cp;
-        // tile element (i, j): H row of pivot P's row i, column of pivot P2's
-        // row j; stored in the upper block tile, else as its conjugate
-        auto hload = [&](int i, int j) {
-            const int gi = i < b ? I * b + i : J * b + (i - b);
-            const int gj = j < b ? I2 * b + j : J2 * b + (j - b);
-            if (gi / b < gj / b) return Hg[(int64_t)gi * Cp + gj];
-            const double2 t = Hg[(int64_t)gj * Cp + gi];
-            return make_double2(t.x, -t.y);
-        };
-        auto hstore = [&](int i, int j, double2 v) {
-            const int gi = i < b ? I * b + i : J * b + (i - b);
-            const int gj = j < b ? I2 * b + j : J2 * b + (j - b);
-            if (gi / b < gj / b)
-                Hg[(int64_t)gi * Cp + gj] = v;
+        // The tile's four 16x16 block tiles (row block a of P: I or J, column
+        // block c of P2: I2 or J2) are stored as H[bi][bj] when bi < bj, else
+        // as the conjugate transpose in H[bj][bi]; both are read and written
+        // row-contiguously (256 B per block row) and transposed in LDS.
+        const int ii = tid >> 4, jj = tid & 15;  // one element of each block tile per thread
+        int64_t gaddr[4];
+        bool lower[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int bi = (t >> 1) ? J : I, bj = (t & 1) ? J2 : I2;
+            lower[t] = bi > bj;
+            gaddr[t] = lower[t] ? (int64_t)(bj * b + ii) * Cp + bi * b + jj
+                                : (int64_t)(bi * b + ii) * Cp + bj * b + jj;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const double2 v = Hg[gaddr[t]];
+            const int r0 = 16 * (t >> 1), c0 = 16 * (t & 1);
+            if (lower[t])
+                Ts[r0 + jj][c0 + ii] = make_double2(v.x, -v.y);
             else
-                Hg[(int64_t)gj * Cp + gi] = make_double2(v.x, -v.y);
-        };
+                Ts[r0 + ii][c0 + jj] = v;
+        }
+        __syncthreads();
         const int tm = wave >> 1, tn = wave & 1;
         // W = Q_P^H T: register r of lane (l16, lq) is W[16 tm + 4 r + lq][16 tn + l16]
-        double2 w[4];
         if (fa) {
             const double2 *Qa = Qall + (int64_t)P * 1024;
             mfma_d4 a1 = {0.0, 0.0, 0.0, 0.0}, a2 = a1, a3 = a1;
@@ -1088,40 +1089,48 @@ k_hupdate(const mpsx_two_site_op *ops, const int *owner, char *ws, int round) {
             for (int s = 0; s < 8; ++s) {
                 const int k = 4 * s + lq;
                 const double2 q = Qa[k * 32 + 16 * tm + l16];  // conj: A = Q^H[16 tm + l16][k]
-                const double2 t = hload(k, 16 * tn + l16);     // B = T[k][16 tn + l16]
+                const double2 t = Ts[k][16 * tn + l16];        // B = T[k][16 tn + l16]
                 a1 = mfma16d(q.x, t.x, a1);
                 a2 = mfma16d(q.y, t.y, a2);
                 a3 = mfma16d(q.x - q.y, t.x + t.y, a3);
             }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) w[r] = make_double2(a1[r] + a2[r], a3[r] - a1[r] + a2[r]);
-        } else {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) w[r] = hload(16 * tm + 4 * r + lq, 16 * tn + l16);
+            for (int r = 0; r < 4; ++r)
+                Ws[16 * tm + 4 * r + lq][16 * tn + l16] =
+                    make_double2(a1[r] + a2[r], a3[r] - a1[r] + a2[r]);
+            __syncthreads();
         }
-        if (!fb) {
+        if (fb) {
+            const double2(*Wsrc)[33] = fa ? Ws : Ts;
+            const double2 *Qb = Qall + (int64_t)P2 * 1024;
+            mfma_d4 c1 = {0.0, 0.0, 0.0, 0.0}, c2 = c1, c3 = c1;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) hstore(16 * tm + 4 * r + lq, 16 * tn + l16, w[r]);
-            return;
+            for (int s = 0; s < 8; ++s) {
+                const int k = 4 * s + lq;
+                const double2 wa = Wsrc[16 * tm + l16][k];         // A = W[16 tm + l16][k]
+                const double2 qb = Qb[k * 32 + 16 * tn + l16];     // B = Q_P2[k][16 tn + l16]
+                c1 = mfma16d(wa.x, qb.x, c1);
+                c2 = mfma16d(wa.y, qb.y, c2);
+                c3 = mfma16d(wa.x + wa.y, qb.x + qb.y, c3);
+            }
+            __syncthreads();  // every wave has read Ts (as W) before it is overwritten
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                Ts[16 * tm + 4 * r + lq][16 * tn + l16] =
+                    make_double2(c1[r] - c2[r], c3[r] - c1[r] - c2[r]);
+            __syncthreads();
         }
+        const double2(*Out)[33] = fb ? Ts : Ws;  // fa only: the result is W
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Ws[16 * tm + 4 * r + lq][16 * tn + l16] = w[r];
-        __syncthreads();
-        const double2 *Qb = Qall + (int64_t)P2 * 1024;
-        mfma_d4 c1 = {0.0, 0.0, 0.0, 0.0}, c2 = c1, c3 = c1;
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const int k = 4 * s + lq;
-            const double2 wa = Ws[16 * tm + l16][k];           // A = W[16 tm + l16][k]
-            const double2 qb = Qb[k * 32 + 16 * tn + l16];     // B = Q_P2[k][16 tn + l16]
-            c1 = mfma16d(wa.x, qb.x, c1);
-            c2 = mfma16d(wa.y, qb.y, c2);
-            c3 = mfma16d(wa.x + wa.y, qb.x + qb.y, c3);
+        for (int t = 0; t < 4; ++t) {
+            const int r0 = 16 * (t >> 1), c0 = 16 * (t & 1);
+            if (lower[t]) {
+                const double2 v = Out[r0 + jj][c0 + ii];
+                Hg[gaddr[t]] = make_double2(v.x, -v.y);
+            } else {
+                Hg[gaddr[t]] = Out[r0 + ii][c0 + jj];
+            }
         }
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-            hstore(16 * tm + 4 * r + lq, 16 * tn + l16,
-                   make_double2(c1[r] - c2[r], c3[r] - c1[r] - c2[r]));
         return;
     }
     // Y rows of pivot P, 16 columns per wave
@@ -1447,7 +1456,7 @@ hipError_t launch_hgram(hipStream_t s, const mpsx_two_site_op *d, const int *own
 hipError_t launch_hsolve(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
                          int64_t pairs, int round, double tol, double tol_inner, int inner_sweeps,
                          double abs_rel) {
-    hipLaunchKernelGGL(k_hsolve, dim3((unsigned)pairs), dim3(256), 0, s, d, owner, ws, round, tol,
+    hipLaunchKernelGGL(k_hsolve, dim3((unsigned)pairs), dim3(HSOLVE_THREADS), 0, s, d, owner, ws, round, tol,
                        tol_inner, inner_sweeps, abs_rel);
     return hipGetLastError();
 }

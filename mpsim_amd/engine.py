@@ -122,7 +122,7 @@ class Engine:
         self.svd = tuned_config()
         self.last_sweeps = 0
         self.split_min = 8          # problems with >= SPLIT_ROWS rows needed to split a call
-        self.split_ways = 2         # concurrent groups of a split call
+        self.split_ways = 4         # concurrent groups of a split call (4 hardware queues)
         self._wss: list = []        # workspaces of the helper streams' groups
         self._helpers: list = []    # (stream, single-thread executor) per helper group
         self._keep: list = []       # device tables of the last call, alive until the next
