@@ -233,8 +233,8 @@ def run_qutrit(args, eng, svd):
     depth = args.depth or 16
     circ = brickwork_haar(48, depth, 3, maxsvals=1024, seed=4)
     flops, _, updates = circuit_work(circ)
-    svd_f, _ = svd_flops(circ, svd.block_switch or 64, svd.fp32)
-    kname = ("k_jacobi_round32" if svd.fp32 else "k_jacobi_round64") + "<32>"
+    svd_f, _ = svd_flops(circ, svd.block_switch or 64)
+    kset = round_kernels(svd)
     peak = PEAK_FP32_MFMA if svd.fp32 else PEAK_FP64_MFMA
     dc = prepare(circ, eng)
     torch.cuda.synchronize()
@@ -245,15 +245,18 @@ def run_qutrit(args, eng, svd):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     nat.profile_enable(False)
+    busy = nat.profile_busy(kset)
     kern = nat.profile_read(reset=True)
-    jr_ms, jr_n = kern.get(kname, (0.0, 0))
-    ach = svd_f[kname] / (jr_ms * 1e-3) / 1e12 if jr_ms else 0.0
+    jr_n = kern.get(kset[0], (0.0, 0))[1]
+    ach = svd_f["big"] / (busy * 1e-3) / 1e12 if busy else 0.0
     line = base_line("2-site gate updates/s, 48 qutrits chi<=1024", updates / dt, "2-site updates/s",
                      args, 1e3 * dt, {"workload": f"48 qutrits, brickwork depth {depth}, 9x9 Haar, "
                                       "maxsvals=1024", "updates": updates})
-    line["roofline"] = {"kernel": kname, "bound": "mfma", "achieved": round(ach, 3),
+    line["roofline"] = {"kernel": "+".join(kset), "bound": "mfma", "achieved": round(ach, 3),
                         "peak": peak, "unit": "TFLOP/s",
-                        "frac": round(ach / peak, 5), "launches": jr_n}
+                        "frac": round(ach / peak, 5), "rounds": jr_n,
+                        "achieved_basis": "SVD flops of the updates with 32-row pivots / busy time "
+                                          "of their round kernels"}
     line["extra"] = {"kernel_ms": {k: round(v[0], 2) for k, v in sorted(kern.items())},
                      "bond_dimensions_max": max(dc.mps.bond_dimensions()),
                      "algorithmic_tflops_per_s": round(flops / dt / 1e12, 3)}
@@ -359,16 +362,24 @@ def run_served(args, eng, svd):
     emit(line)
 
 
-def svd_flops(circuit, switch=64, fp32=False):
-    """Golub-Reinsch SVD flops (the SVD term of the per-update work), split by
-    the Jacobi kernel that runs the update (32-row pivots when the rows to
-    orthogonalise, d*chi_l, exceed `switch`), and the full per-update flops,
-    summed over the circuit. Returns ({kernel: flops}, full)."""
+def round_kernels(svd):
+    """Kernels of one Jacobi round of the updates with 32-row pivots: the
+    eigensolve and update launches of the Gram method (FP64 default), else
+    the one-sided round kernel."""
+    if not svd.fp32 and svd.method == 0:
+        return ["k_hsolve", "k_hupdate"]
+    return [("k_jacobi_round32" if svd.fp32 else "k_jacobi_round64") + "<32>"]
+
+
+def svd_flops(circuit, switch=64):
+    """Golub-Reinsch SVD flops (the SVD term of the per-update work) of the
+    updates with 32-row pivots (the rows to orthogonalise, d*chi_l, exceed
+    `switch`) and of the others, summed over the circuit, and the full
+    per-update flops. Returns ({"big": flops, "small": flops}, full)."""
     from mpsim_amd.circuits import two_site_flops
     n, d = circuit.n, circuit.d
     bonds = [1] * (n + 1)
-    pre = "k_jacobi_round32" if fp32 else "k_jacobi_round64"
-    svd = {pre + "<16>": 0.0, pre + "<32>": 0.0}
+    svd = {"big": 0.0, "small": 0.0}
     full = 0.0
     for layer in circuit.layers:
         for q, _ in layer:
@@ -380,8 +391,7 @@ def svd_flops(circuit, switch=64, fp32=False):
             f = two_site_flops(cl, cm, cr, d, k)
             M, N = d * cl, d * cr
             p, qq = max(M, N), min(M, N)
-            kern = pre + ("<32>" if M > switch else "<16>")
-            svd[kern] += 4.0 * (4.0 * p * p * qq + 8.0 * p * qq * qq + 9.0 * qq ** 3)
+            svd["big" if M > switch else "small"] += 4.0 * (4.0 * p * p * qq + 8.0 * p * qq * qq + 9.0 * qq ** 3)
             full += f
             bonds[i + 1] = k
     return svd, full
@@ -433,7 +443,8 @@ def main():
         return run_qutrit(args, eng, svd)
     circ = brickwork_haar(64, 20, 2, maxsvals=256, seed=2)
     flops, bytes_, updates = circuit_work(circ)
-    svd_f, _ = svd_flops(circ, svd.block_switch or 64, svd.fp32)
+    svd_f, _ = svd_flops(circ, svd.block_switch or 64)
+    kset = round_kernels(svd)
 
     dc = prepare(circ, eng)
     for _ in range(args.warmup):
@@ -452,25 +463,31 @@ def main():
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
     nat.profile_enable(False)
+    jr_busy = nat.profile_busy(kset)
     full = nat.profile_read_full(reset=True)
     kern = {k: (v["total_ms"], v["launches"]) for k, v in full.items()}
     total = sum(times)
     value = updates * args.steps / total
     ms_per_step = 1e3 * total / args.steps
 
-    dom = max(kern.items(), key=lambda kv: kv[1][0])[0] if kern else ""
-    jr_ms, jr_n = kern.get(dom, (0.0, 0))
-    jr_busy = full.get(dom, {}).get("busy_ms", 0.0)
-    dom_f = svd_f.get(dom, 0.0) * args.steps
-    # Large two-site calls run as two concurrent halves on two streams, so the
-    # kernel's launches overlap in pairs: its throughput is its algorithmic
-    # flops over its busy time (the union of its launch intervals); the
-    # per-launch figure (flops per launch / average launch duration) is
+    # The roofline unit is one Jacobi round of the updates with 32-row pivots
+    # (k_hsolve + k_hupdate for the Gram method, one k_jacobi_round launch
+    # otherwise). Large two-site calls run as four concurrent groups on four
+    # streams, so rounds overlap: throughput is the SVD flops over the busy
+    # time of the round kernels (the union of all their launch intervals);
+    # the per-round figure (flops per round / average round duration) is
     # reported beside it.
+    dom = "+".join(kset)
+    jr_ms = sum(kern.get(k, (0.0, 0))[0] for k in kset)
+    jr_n = kern.get(kset[0], (0.0, 0))[1]
+    dom_f = svd_f["big"] * args.steps
     achieved = dom_f / (jr_busy * 1e-3) / 1e12 if jr_busy > 0 else 0.0
-    per_launch = dom_f / (jr_ms * 1e-3) / 1e12 if jr_ms > 0 else 0.0
+    per_round = dom_f / (jr_ms * 1e-3) / 1e12 if jr_ms > 0 else 0.0
     traffic = load_traffic()
-    # executed FP64 MFMA flops per launch (SQ_INSTS_VALU_MFMA_MOPS_F64 x 512 from a
+    tk = (traffic or {}).get("kernels", {})
+    tbytes = (sum(tk[k]["bytes_per_launch"] for k in kset)
+              if traffic and all(k in tk for k in kset) else None)
+    # executed FP64 MFMA flops per round (SQ_INSTS_VALU_MFMA_MOPS_F64 x 512 from a
     # counter pass) over the live busy time
     mc = (traffic or {}).get("mfma_counters")
     roofline = {
@@ -480,25 +497,28 @@ def main():
         "peak": peak,
         "unit": "TFLOP/s",
         "frac": round(achieved / peak, 5),
-        "traffic": (traffic.get("kernels", {}).get(dom, {}).get("bytes_per_launch")
-                    if traffic else None),
-        "achieved_basis": "algorithmic SVD flops of the kernel's launches / its busy time "
-                          "(union of launch intervals, HIP events on the launch streams)",
+        "traffic": tbytes,
+        "traffic_unit": "HBM bytes per round (calibrated FETCH_SIZE + WRITE_SIZE)",
+        "achieved_basis": "algorithmic SVD flops of the updates with 32-row pivots / busy time of "
+                          "their round kernels (union of launch intervals, HIP events on the launch "
+                          "streams)",
         "busy_ms": round(jr_busy, 3),
         "concurrency": round(jr_ms / jr_busy, 3) if jr_busy > 0 else None,
-        "achieved_per_launch": round(per_launch, 3),
-        "launches": jr_n,
-        "avg_launch_us": round(1e3 * jr_ms / max(jr_n, 1), 3),
+        "achieved_per_round": round(per_round, 3),
+        "rounds": jr_n,
+        "avg_round_us": round(1e3 * jr_ms / max(jr_n, 1), 3),
+        "avg_launch_us": {k: round(1e3 * kern[k][0] / max(kern[k][1], 1), 3)
+                          for k in kset if k in kern},
         "executed_mfma_frac_counters": (
-            round(mc["executed_flop_per_launch"] * jr_n / (jr_busy * 1e-3) / 1e12 / peak, 4)
+            round(mc["executed_flop_per_round"] * jr_n / (jr_busy * 1e-3) / 1e12 / peak, 4)
             if (mc and not svd.fp32 and jr_busy > 0 and mc.get("kernel") == dom) else None),
-        "algorithmic_flops_per_launch": dom_f / max(jr_n, 1),
-        "algorithmic_unit": "SVD term 4(4p^2q+8pq^2+9q^3) of the updates this kernel runs",
+        "algorithmic_flops_per_round": dom_f / max(jr_n, 1),
+        "algorithmic_unit": "SVD term 4(4p^2q+8pq^2+9q^3) of the updates with 32-row pivots",
     }
 
     extra = {
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in sorted(kern.items())},
-        "dominant_kernel": dom,
+        "round_kernels": dom,
         "sweeps_last_layer": eng.last_sweeps,
         "algorithmic_tflops_per_s": round(flops * args.steps / total / 1e12, 3),
         "algorithmic_hbm_gb_per_s": round(bytes_ * args.steps / total / 1e9, 3),

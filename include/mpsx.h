@@ -223,6 +223,10 @@ int mpsx_profile_enable(int on);
 /* Writes up to `cap` entries; returns the number of distinct kernels. */
 int mpsx_profile_read(char *names, int name_len, double *total_ms, double *busy_ms,
                       int64_t *launches, int cap, int reset);
+/* Busy time (ms) of a group of kernels since the last reset: the length of
+ * the union of the launch intervals of every kernel named in `names`
+ * (comma-separated), e.g. the two launches of a Jacobi round. */
+int mpsx_profile_busy(const char *names, double *busy_ms);
 
 /* ------------------------------------------------------------------------
  * Graph capture of library calls. Every call above launches kernels only

@@ -63,8 +63,8 @@ class NotConvergedWarning(RuntimeWarning):
 
 EXPORTS = ("mpsx_version", "mpsx_last_error", "mpsx_device_info", "mpsx_sizeof",
            "mpsx_apply_one_site", "mpsx_two_site_plan", "mpsx_two_site", "mpsx_gemm",
-           "mpsx_scale", "mpsx_profile_enable", "mpsx_profile_read", "mpsx_graph_begin",
-           "mpsx_graph_end", "mpsx_graph_launch", "mpsx_graph_destroy")
+           "mpsx_scale", "mpsx_profile_enable", "mpsx_profile_read", "mpsx_profile_busy",
+           "mpsx_graph_begin", "mpsx_graph_end", "mpsx_graph_launch", "mpsx_graph_destroy")
 
 _lock = threading.Lock()
 _lib: Optional[ctypes.CDLL] = None
@@ -96,13 +96,14 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
         lib.mpsx_profile_read.argtypes = [ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_int64), i32, i32]
+        lib.mpsx_profile_busy.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
         lib.mpsx_graph_begin.argtypes = [vp]
         lib.mpsx_graph_end.argtypes = [vp, ctypes.POINTER(vp)]
         lib.mpsx_graph_launch.argtypes = [vp, vp]
         lib.mpsx_graph_destroy.argtypes = [vp]
         for name in ("mpsx_apply_one_site", "mpsx_two_site_plan", "mpsx_two_site", "mpsx_gemm",
                      "mpsx_scale", "mpsx_device_info", "mpsx_profile_enable",
-                     "mpsx_profile_read", "mpsx_graph_begin", "mpsx_graph_end",
+                     "mpsx_profile_read", "mpsx_profile_busy", "mpsx_graph_begin", "mpsx_graph_end",
                      "mpsx_graph_launch", "mpsx_graph_destroy"):
             getattr(lib, name).restype = i32
         sizes = [lib.mpsx_sizeof(i) for i in range(6)]
@@ -180,6 +181,15 @@ def profile_read_full(reset: bool = True) -> dict:
         nm = names.raw[i * nlen:(i + 1) * nlen].split(b"\0", 1)[0].decode()
         out[nm] = {"total_ms": float(tot[i]), "busy_ms": float(busy[i]), "launches": int(cnt[i])}
     return out
+
+
+def profile_busy(names) -> float:
+    """Busy time (ms) of a group of kernels (union of all their launch
+    intervals) since the last reset; does not reset."""
+    lib = load()
+    out = ctypes.c_double(0.0)
+    check(lib.mpsx_profile_busy(",".join(names).encode(), ctypes.byref(out)))
+    return float(out.value)
 
 
 def profile_read(reset: bool = True) -> dict:

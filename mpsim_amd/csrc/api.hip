@@ -510,6 +510,53 @@ int mpsx_profile_enable(int on) {
     return MPSX_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Length of the union of intervals (sorted in place).
+double union_length(std::vector<std::pair<double, double>> &sp) {
+    std::sort(sp.begin(), sp.end());
+    double busy = 0.0, lo = 0.0, hi = -1.0;
+    for (const auto &iv : sp) {
+        if (iv.first > hi) {
+            if (hi > lo) busy += hi - lo;
+            lo = iv.first;
+            hi = iv.second;
+        } else if (iv.second > hi) {
+            hi = iv.second;
+        }
+    }
+    if (hi > lo) busy += hi - lo;
+    return busy;
+}
+}  // namespace
+
+extern "C" {
+
+int mpsx_profile_busy(const char *names, double *busy_ms) {
+    if (!names || !busy_ms) return fail(MPSX_EINVAL, "null argument");
+    std::vector<std::string> want;
+    for (const char *a = names; *a;) {
+        const char *b = std::strchr(a, ',');
+        const size_t n = b ? (size_t)(b - a) : std::strlen(a);
+        if (n) want.emplace_back(a, n);
+        a += n + (b ? 1 : 0);
+    }
+    Profiler &p = prof();
+    std::lock_guard<std::mutex> g(p.mu);
+    std::vector<std::pair<double, double>> spans;
+    for (const ProfRec &r : p.recs) {
+        if (std::find(want.begin(), want.end(), std::string(r.name)) == want.end()) continue;
+        HIP_TRY(hipEventSynchronize(r.b), "hipEventSynchronize");
+        float ms = 0.f, t0 = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
+        HIP_TRY(hipEventElapsedTime(&t0, p.recs.front().a, r.a), "hipEventElapsedTime");
+        spans.emplace_back((double)t0, (double)t0 + ms);
+    }
+    *busy_ms = union_length(spans);
+    return MPSX_OK;
+}
+
 int mpsx_profile_read(char *names, int name_len, double *total_ms, double *busy_ms,
                       int64_t *launches, int cap, int reset) {
     Profiler &p = prof();
@@ -539,22 +586,8 @@ int mpsx_profile_read(char *names, int name_len, double *total_ms, double *busy_
             }
             if (total_ms) total_ms[i] = kv.second.total;
             if (launches) launches[i] = kv.second.n;
-            if (busy_ms) {  // union of the launch intervals (launches on several streams overlap)
-                auto &sp = kv.second.spans;
-                std::sort(sp.begin(), sp.end());
-                double busy = 0.0, lo = 0.0, hi = -1.0;
-                for (const auto &iv : sp) {
-                    if (iv.first > hi) {
-                        if (hi > lo) busy += hi - lo;
-                        lo = iv.first;
-                        hi = iv.second;
-                    } else if (iv.second > hi) {
-                        hi = iv.second;
-                    }
-                }
-                if (hi > lo) busy += hi - lo;
-                busy_ms[i] = busy;
-            }
+            // union of the launch intervals (launches on several streams overlap)
+            if (busy_ms) busy_ms[i] = union_length(kv.second.spans);
         }
         ++i;
     }

@@ -177,3 +177,47 @@ def test_simulator_rejects_bad_programs():
     from mpsim_amd.mpsim_cirq import MPSimulator
     with pytest.raises(ValueError):
         MPSimulator().simulate(42)
+
+
+def _rr_pair(p, r, n):
+    """The round-robin pairing of common.h (rr_item / rr_pair)."""
+    def item(pos):
+        return 0 if pos == 0 else ((pos - 1 + r) % (n - 1)) + 1
+    a, b = item(p), item(n - 1 - p)
+    return (min(a, b), max(a, b))
+
+
+@pytest.mark.parametrize("nb", [2, 4, 6, 10, 32, 34, 192])
+def test_gram_tile_enumeration(nb):
+    """The Gram method's work split (k_hgram, k_hsolve, k_hupdate in
+    twosite.hip): k_hgram's (slot j, y) grid enumerates every upper 16x16 block
+    tile of H once, and in every round the tiles k_hsolve writes (the three of
+    each pivot) and those k_hupdate writes (the four of each unordered pair of
+    pivots, pair (P, P + y + 1 mod n) for y < n/2, P < n/2 when 2(y + 1) = n)
+    cover every stored tile exactly once."""
+    n = nb // 2
+    upper = {(i, j) for i in range(nb) for j in range(i, nb)}
+    seen = []
+    for j in range(n):
+        for y in range(nb + 1):
+            rem, bi = j * (nb + 1) + y, 0
+            while rem >= nb - bi:
+                rem -= nb - bi
+                bi += 1
+            seen.append((bi, bi + rem))
+    assert sorted(seen) == sorted(upper)
+    for r in range(nb - 1):
+        piv = [_rr_pair(P, r, nb) for P in range(n)]
+        assert sorted(x for pr in piv for x in pr) == list(range(nb))
+        written = []
+        for P in range(n):
+            I, J = piv[P]
+            written += [(I, I), (I, J), (J, J)]
+            for y in range(n // 2):
+                dd = y + 1
+                if 2 * dd == n and P >= n // 2:
+                    continue
+                for bi in piv[P]:
+                    for bj in piv[(P + dd) % n]:
+                        written.append((min(bi, bj), max(bi, bj)))
+        assert sorted(written) == sorted(upper)
