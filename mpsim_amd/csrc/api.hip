@@ -99,8 +99,11 @@ struct Profiler {
             pool.pop_back();
             return e;
         }
+        // timing-only events: no system-scope fence (cache writeback and
+        // invalidation) when they are recorded, which would otherwise stall
+        // the stream between the kernels being timed
         hipEvent_t e = nullptr;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
         return e;
     }
 };
