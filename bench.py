@@ -249,7 +249,7 @@ def run_qutrit(args, eng, svd):
     nat.profile_enable(False)
     busy = nat.profile_busy(kset)
     kern = nat.profile_read(reset=True)
-    jr_n = kern.get(kset[0], (0.0, 0))[1]
+    jr_n = sum(kern.get(k, (0.0, 0))[1] for k in kset[:2]) if len(kset) == 3 else kern.get(kset[0], (0.0, 0))[1]
     ach = svd_f["big"] / (busy * 1e-3) / 1e12 if busy else 0.0
     line = base_line("2-site gate updates/s, 48 qutrits chi<=1024", updates / dt, "2-site updates/s",
                      args, 1e3 * dt, {"workload": f"48 qutrits, brickwork depth {depth}, 9x9 Haar, "
@@ -369,7 +369,7 @@ def round_kernels(svd):
     eigensolve and update launches of the Gram method (FP64 default), else
     the one-sided round kernel."""
     if not svd.fp32 and svd.method == 0:
-        return ["k_hsolve", "k_hupdate"]
+        return ["k_hsolve", "k_hround", "k_hupdate"]
     return [("k_jacobi_round32" if svd.fp32 else "k_jacobi_round64") + "<32>"]
 
 
@@ -491,7 +491,9 @@ def main():
     # reported beside it.
     dom = "+".join(kset)
     jr_ms = sum(kern.get(k, (0.0, 0))[0] for k in kset)
-    jr_n = kern.get(kset[0], (0.0, 0))[1]
+    # rounds: one eigensolve launch each (k_hsolve opens a sweep, k_hround runs
+    # the next round's eigensolves beside the current round's update)
+    jr_n = sum(kern.get(k, (0.0, 0))[1] for k in kset[:2]) if len(kset) == 3 else kern.get(kset[0], (0.0, 0))[1]
     dom_f = svd_f["big"] * args.steps
     achieved = dom_f / (jr_busy * 1e-3) / 1e12 if jr_busy > 0 else 0.0
     per_round = dom_f / (jr_ms * 1e-3) / 1e12 if jr_ms > 0 else 0.0

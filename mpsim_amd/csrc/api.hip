@@ -348,7 +348,7 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
     // Re-derive totals from the planned host table and check the plan is
     // consistent (a table that was not planned, or edited, is rejected).
     int64_t need = kHeaderBytes, tiles = 0, pairs = 0, rows = 0, outs = 0;
-    int rows_nb = 0, gram_nb = 0, gram_y = 0, gram_c = 0;
+    int rows_nb = 0, gram_nb = 0, gram_y = 0, gram_c = 0, gram_nt = 0;
     bool has_d[5] = {false, false, false, false, false};  // index 1: plain splits
     bool has_b[17] = {false};  // one-sided problems by block width
     bool has_gram = false;
@@ -378,6 +378,7 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
             has_gram = true;
             gram_nb = std::max(gram_nb, p.nb);
             gram_y = std::max(gram_y, p.nb / 4 + (p.Cp + 63) / 64);
+            gram_nt = std::max(gram_nt, std::max(p.nb / 4, 1));
             gram_c = std::max(gram_c, (p.R + 31) / 32);
         }
     }
@@ -420,15 +421,33 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
                                                                      r.tol, r.tol * 0.01, r.inner_sweeps);
                                       }),
                                 "k_jacobi_round");
-            for (int round = 0; has_gram && round < gram_nb - 1; ++round) {
+            // Gram rounds 0 .. R-1 (R = gram_nb - 1), pipelined on the stream:
+            //   k_hsolve(0); then for r < R-1: k_hupdate(r, tiles the
+            //   eigensolves of r+1 read), k_hround(eigensolves of r+1 beside
+            //   the rest of round r's update); finally k_hupdate(R-1, whole).
+            if (has_gram) {
+                const double ti = r.tol * 0.01;
                 HIP_TRY(timed("k_hsolve", stream,
                               [&] {
-                                  return launch_hsolve(stream, d, owner, ws, pairs, round, r.tol, r.tol * 0.01,
-                                                       r.inner_sweeps, kGramAbsRel);
+                                  return launch_hsolve(stream, d, owner, ws, pairs, 0, r.tol, ti, r.inner_sweeps,
+                                                       kGramAbsRel);
                               }),
                         "k_hsolve");
+                for (int round = 0; round < gram_nb - 2; ++round) {
+                    HIP_TRY(timed("k_hupdate", stream,
+                                  [&] { return launch_hupdate(stream, d, owner, ws, pairs, gram_nt, round, 1); }),
+                            "k_hupdate");
+                    HIP_TRY(timed("k_hround", stream,
+                                  [&] {
+                                      return launch_hround(stream, d, owner, ws, pairs, gram_y, round, r.tol, ti,
+                                                           r.inner_sweeps, kGramAbsRel);
+                                  }),
+                            "k_hround");
+                }
                 HIP_TRY(timed("k_hupdate", stream,
-                              [&] { return launch_hupdate(stream, d, owner, ws, pairs, gram_y, round); }),
+                              [&] {
+                                  return launch_hupdate(stream, d, owner, ws, pairs, gram_y, gram_nb - 2, 0);
+                              }),
                         "k_hupdate");
             }
             HIP_TRY(timed("k_sweep_end", stream, [&] { return launch_sweep_end(stream, d, count, ws, s); }), "k_sweep_end");

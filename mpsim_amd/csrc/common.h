@@ -113,9 +113,9 @@ MPSX_HD int64_t perm_off(const mpsx_plan &p) { return sig_off(p) + align_up(8LL 
 MPSX_HD int64_t z_off(const mpsx_plan &p) { return perm_off(p) + align_up(4LL * p.Cp, 256); }
 // Problems of the Gram method (plan.method 1, b = 16, FP64) add after Z:
 //   H[Cp][Cp] (double2, upper 16x16 block tiles valid; at the end the same
-//   region holds a copy of X as Cp x R) | Q[nb/2][32][32] (double2, the
-//   rotations of each pivot in the current round) | flag[nb/2] (int32,
-//   1 when the pivot rotated in the current round)
+//   region holds a copy of X as Cp x R) | Q[2][nb/2][32][32] (double2, the
+//   rotations of each pivot, for even and odd rounds) | flag[2][nb/2]
+//   (int32, 1 when the pivot rotated in that round)
 MPSX_HD int64_t h_elems(const mpsx_plan &p) {
     return (int64_t)p.Cp * (p.Cp > p.R ? p.Cp : p.R);
 }
@@ -123,10 +123,10 @@ MPSX_HD int64_t h_off(const mpsx_plan &p) {
     return z_off(p) + align_up((int64_t)p.zbytes * p.Cp * p.LD, 256);
 }
 MPSX_HD int64_t qmat_off(const mpsx_plan &p) { return h_off(p) + align_up(16 * h_elems(p), 256); }
-MPSX_HD int64_t qflag_off(const mpsx_plan &p) { return qmat_off(p) + 16LL * (p.nb / 2) * 1024; }
+MPSX_HD int64_t qflag_off(const mpsx_plan &p) { return qmat_off(p) + 2 * 16LL * (p.nb / 2) * 1024; }
 MPSX_HD int64_t problem_bytes(const mpsx_plan &p) {
     if (p.method != 1) return align_up(z_off(p) - p.ws_off + (int64_t)p.zbytes * p.Cp * p.LD, 256);
-    return align_up(qflag_off(p) - p.ws_off + 4LL * (p.nb / 2), 256);
+    return align_up(qflag_off(p) - p.ws_off + 2 * 4LL * (p.nb / 2), 256);
 }
 
 // Complex element of the working matrix: float2 (FP32 rotations) or

@@ -36,7 +36,10 @@ hipError_t launch_hsolve(hipStream_t s, const mpsx_two_site_op *d, const int *ow
                          int64_t pairs, int round, double tol, double tol_inner, int inner_sweeps,
                          double abs_rel);
 hipError_t launch_hupdate(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
-                          int64_t pairs, int ytiles, int round);
+                          int64_t pairs, int ytiles, int round, int part);
+hipError_t launch_hround(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+                         int64_t pairs, int ytiles, int round, double tol, double tol_inner,
+                         int inner_sweeps, double abs_rel);
 hipError_t launch_hxcopy(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int64_t rows);
 hipError_t launch_hxgemm(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
                          int64_t pairs, int ctiles);

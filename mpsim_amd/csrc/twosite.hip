@@ -901,11 +901,21 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
 MPSX_DEV double2 *h_ptr(char *ws, const mpsx_plan &p) {
     return reinterpret_cast<double2 *>(ws + h_off(p));
 }
-MPSX_DEV double2 *qmat_ptr(char *ws, const mpsx_plan &p) {
-    return reinterpret_cast<double2 *>(ws + qmat_off(p));
+// Q and flags of round `round` (two sets, so that the update launches of a
+// round may still read its rotations while the next round's eigensolves
+// write theirs).
+MPSX_DEV double2 *qmat_ptr(char *ws, const mpsx_plan &p, int round) {
+    return reinterpret_cast<double2 *>(ws + qmat_off(p)) + (int64_t)(round & 1) * (p.nb / 2) * 1024;
 }
-MPSX_DEV int *qflag_ptr(char *ws, const mpsx_plan &p) {
-    return reinterpret_cast<int *>(ws + qflag_off(p));
+MPSX_DEV int *qflag_ptr(char *ws, const mpsx_plan &p, int round) {
+    return reinterpret_cast<int *>(ws + qflag_off(p)) + (int64_t)(round & 1) * (p.nb / 2);
+}
+
+// Block paired with block `item` in round r of the circle method over n
+// blocks (the inverse of rr_item, then the slot opposite).
+MPSX_DEV int rr_partner(int item, int r, int n) {
+    const int pos = item == 0 ? 0 : ((item - 1 - r) % (n - 1) + (n - 1)) % (n - 1) + 1;
+    return rr_item(n - 1 - pos, r, n);
 }
 
 // One workgroup per upper 16x16 block tile (bi <= bj) of H = X X^H. x: the
@@ -973,15 +983,11 @@ constexpr int HSOLVE_THREADS = 256;
 // FP64 rounding H gathers over the rounds (pairs below it would otherwise
 // keep rotating on noise; their residual coupling changes the truncated
 // factors only to second order).
-__global__ void __launch_bounds__(HSOLVE_THREADS)
-k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
-         double tol_inner, int inner_sweeps, double abs_rel) {
+MPSX_DEV void hsolve_body(const mpsx_two_site_op *ops, const int *owner, char *ws, int round,
+                          double tol, double tol_inner, int inner_sweeps, double abs_rel,
+                          double2 (*H)[33], double2 (*Q)[33], double *prm_c, double2 *prm_s,
+                          int &s_any) {
     constexpr int B2 = 32, b = 16, NT = HSOLVE_THREADS;
-    __shared__ double2 H[B2][B2 + 1];
-    __shared__ double2 Q[B2][B2 + 1];
-    __shared__ double prm_c[b];
-    __shared__ double2 prm_s[b];
-    __shared__ int s_any;
     const int pi = owner[blockIdx.x];
     const mpsx_plan p = ops[pi].plan;
     if (p.method != 1 || round >= p.nb - 1) return;
@@ -1006,13 +1012,13 @@ k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, dou
     }
     __syncthreads();
     const double abs_t = abs_rel * ctl->fro2;
-    int *flag = qflag_ptr(ws, p);
+    int *flag = qflag_ptr(ws, p, round);
     if (!pivot_solve<B2, NT>(H, Q, prm_c, prm_s, s_any, ctl, ctl->null2, tol, tol_inner, inner_sweeps,
                          abs_t * abs_t)) {
         if (tid == 0) flag[pair] = 0;
         return;
     }
-    double2 *Qg = qmat_ptr(ws, p) + (int64_t)pair * B2 * B2;
+    double2 *Qg = qmat_ptr(ws, p, round) + (int64_t)pair * B2 * B2;
     for (int e = tid; e < B2 * B2; e += NT) {
         const int i = e / B2, j = e % B2;
         Qg[e] = Q[i][j];
@@ -1021,7 +1027,21 @@ k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, dou
     if (tid == 0) flag[pair] = 1;
 }
 
-// Round, part 2. x: pivot P of a problem; y < n/2 (n = nb/2 pivots): the
+__global__ void __launch_bounds__(HSOLVE_THREADS)
+k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
+         double tol_inner, int inner_sweeps, double abs_rel) {
+    __shared__ double2 H[32][33];
+    __shared__ double2 Q[32][33];
+    __shared__ double prm_c[16];
+    __shared__ double2 prm_s[16];
+    __shared__ int s_any;
+    hsolve_body(ops, owner, ws, round, tol, tol_inner, inner_sweeps, abs_rel, H, Q, prm_c, prm_s, s_any);
+}
+
+// Round, part 2 (`part` 0: all of it; 1: only the tiles that hold a pivot
+// pair of round + 1, which its eigensolves read; 2: the other tiles and the
+// rows of Y, which may run beside the next round's eigensolves). x: pivot P
+// of a problem; y < n/2 (n = nb/2 pivots): the
 // 32x32 tile of H between P and P2 = (P + y + 1) mod n, each unordered pair
 // once, T <- Q_P^H T Q_P2 (a factor whose pivot did not rotate is the
 // identity and skipped); y >= n/2: columns 64 (y - n/2) .. +63 of the Y
@@ -1029,19 +1049,17 @@ k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, dou
 // complex product; wave w owns output tile (w / 2, w % 2) of the H tile, or
 // 16 columns of all 32 Y rows. Tiles pass through LDS (Ts: T, then the
 // result; Ws: W = Q_P^H T).
-__global__ void __launch_bounds__(256)
-k_hupdate(const mpsx_two_site_op *ops, const int *owner, char *ws, int round) {
+MPSX_DEV void hupdate_body(const mpsx_two_site_op *ops, const int *owner, char *ws, int round,
+                           int part, int y, double2 (*Ts)[33], double2 (*Ws)[33]) {
     constexpr int b = 16;
-    __shared__ double2 Ts[32][33];
-    __shared__ double2 Ws[32][33];
     const int pi = owner[blockIdx.x];
     const mpsx_plan p = ops[pi].plan;
     if (p.method != 1 || round >= p.nb - 1) return;
     if (ctl_ptr(ws, p)->conv) return;
     const int n = p.nb / 2, nt = n / 2;
-    const int P = (int)(blockIdx.x - p.pair_base), y = (int)blockIdx.y;
-    const int *flag = qflag_ptr(ws, p);
-    const double2 *Qall = qmat_ptr(ws, p);
+    const int P = (int)(blockIdx.x - p.pair_base);
+    const int *flag = qflag_ptr(ws, p, round);
+    const double2 *Qall = qmat_ptr(ws, p, round);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     int I, J;
@@ -1054,6 +1072,11 @@ k_hupdate(const mpsx_two_site_op *ops, const int *owner, char *ws, int round) {
         if (!fa && !fb) return;
         int I2, J2;
         rr_pair(P2, round, p.nb, &I2, &J2);
+        if (part != 0) {  // is one of the four block tiles a pivot pair of the next round?
+            const int pi_ = rr_partner(I, round + 1, p.nb), pj_ = rr_partner(J, round + 1, p.nb);
+            const bool crit = pi_ == I2 || pi_ == J2 || pj_ == I2 || pj_ == J2;
+            if (crit != (part == 1)) return;
+        }
         double2 *Hg = h_ptr(ws, p);
         const int Cp = p.Cp;
         // The tile's four 16x16 block tiles (row block a of P: I or J, column
@@ -1134,6 +1157,7 @@ k_hupdate(const mpsx_two_site_op *ops, const int *owner, char *ws, int round) {
         return;
     }
     // Y rows of pivot P, 16 columns per wave
+    if (part == 1) return;
     const int c0 = (y - nt) * 64;
     if (c0 >= p.Cp || !flag[P]) return;
     const double2 *Qa = Qall + (int64_t)P * 1024;
@@ -1168,6 +1192,34 @@ k_hupdate(const mpsx_two_site_op *ops, const int *owner, char *ws, int round) {
             }
         }
     }
+}
+
+__global__ void __launch_bounds__(256)
+k_hupdate(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, int part) {
+    __shared__ double2 Ts[32][33];
+    __shared__ double2 Ws[32][33];
+    hupdate_body(ops, owner, ws, round, part, (int)blockIdx.y, Ts, Ws);
+}
+
+// Rounds r + 1 and r in one launch: y = 0 runs the eigensolves of round
+// r + 1, y >= 1 the part-2 update of round r (its tiles that no eigensolve of
+// round r + 1 reads, and the rows of Y). The two touch disjoint tiles of H
+// and different sets of Q and flags, so they overlap freely; the part-1
+// update of round r (launched before) has written every tile the
+// eigensolves read. LDS is shared between the two roles.
+__global__ void __launch_bounds__(256)
+k_hround(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
+         double tol_inner, int inner_sweeps, double abs_rel) {
+    __shared__ double2 A[32][33];
+    __shared__ double2 B[32][33];
+    __shared__ double prm_c[16];
+    __shared__ double2 prm_s[16];
+    __shared__ int s_any;
+    if (blockIdx.y == 0)
+        hsolve_body(ops, owner, ws, round + 1, tol, tol_inner, inner_sweeps, abs_rel, A, B, prm_c, prm_s,
+                    s_any);
+    else
+        hupdate_body(ops, owner, ws, round, 2, (int)blockIdx.y - 1, A, B);
 }
 
 // End of the Gram method, 1: copy X (Cp x R) into the H region (row stride
@@ -1461,10 +1513,19 @@ hipError_t launch_hsolve(hipStream_t s, const mpsx_two_site_op *d, const int *ow
     return hipGetLastError();
 }
 
+hipError_t launch_hround(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+                         int64_t pairs, int ytiles, int round, double tol, double tol_inner,
+                         int inner_sweeps, double abs_rel) {
+    static_assert(HSOLVE_THREADS == 256, "k_hround runs both roles with one workgroup size");
+    hipLaunchKernelGGL(k_hround, dim3((unsigned)pairs, (unsigned)(ytiles + 1)), dim3(256), 0, s, d, owner, ws,
+                       round, tol, tol_inner, inner_sweeps, abs_rel);
+    return hipGetLastError();
+}
+
 hipError_t launch_hupdate(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
-                          int64_t pairs, int ytiles, int round) {
+                          int64_t pairs, int ytiles, int round, int part) {
     hipLaunchKernelGGL(k_hupdate, dim3((unsigned)pairs, (unsigned)ytiles), dim3(256), 0, s, d, owner,
-                       ws, round);
+                       ws, round, part);
     return hipGetLastError();
 }
 

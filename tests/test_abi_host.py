@@ -221,3 +221,40 @@ def test_gram_tile_enumeration(nb):
                     for bj in piv[(P + dd) % n]:
                         written.append((min(bi, bj), max(bi, bj)))
         assert sorted(written) == sorted(upper)
+
+
+def _rr_partner(item, r, n):
+    """rr_partner of twosite.hip: the inverse of rr_item, then the opposite slot."""
+    pos = 0 if item == 0 else ((item - 1 - r) % (n - 1)) + 1
+    opp = n - 1 - pos
+    return 0 if opp == 0 else ((opp - 1 + r) % (n - 1)) + 1
+
+
+@pytest.mark.parametrize("nb", [4, 6, 10, 32, 34, 192])
+def test_gram_pipelined_round_split(nb):
+    """The pipelined Gram round: the update tiles k_hupdate marks critical
+    (part 1: some block tile of the pivot-pair tile is a pivot pair of the
+    next round, found with rr_partner) together with the pivot tiles k_hsolve
+    writes hold every tile the next round's eigensolves read, so the rest
+    (part 2) may run beside them."""
+    n = nb // 2
+    for r in range(2 * (nb - 1)):  # rounds past nb - 2 wrap to the next sweep's pairing
+        piv = [_rr_pair(P, r % (nb - 1), nb) for P in range(n)]
+        nxt = [_rr_pair(P, (r + 1) % (nb - 1), nb) for P in range(n)]
+        for I, J in nxt:
+            assert _rr_partner(I, r + 1, nb) == J and _rr_partner(J, r + 1, nb) == I
+        have = set()
+        for I, J in piv:
+            have |= {(I, I), (I, J), (J, J)}
+        for P in range(n):
+            I, J = piv[P]
+            for y in range(n // 2):
+                dd = y + 1
+                if 2 * dd == n and P >= n // 2:
+                    continue
+                I2, J2 = piv[(P + dd) % n]
+                a, b = _rr_partner(I, r + 1, nb), _rr_partner(J, r + 1, nb)
+                if a in (I2, J2) or b in (I2, J2):
+                    have |= {(min(x, z), max(x, z)) for x in (I, J) for z in (I2, J2)}
+        for I, J in nxt:
+            assert {(I, I), (I, J), (J, J)} <= have, (nb, r, I, J)
