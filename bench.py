@@ -498,9 +498,8 @@ def main():
     achieved = dom_f / (jr_busy * 1e-3) / 1e12 if jr_busy > 0 else 0.0
     per_round = dom_f / (jr_ms * 1e-3) / 1e12 if jr_ms > 0 else 0.0
     traffic = load_traffic()
-    tk = (traffic or {}).get("kernels", {})
-    tbytes = (sum(tk[k]["bytes_per_launch"] for k in kset)
-              if traffic and all(k in tk for k in kset) else None)
+    tbytes = (traffic.get("bytes_per_round")
+              if traffic and traffic.get("kernel") == "+".join(kset) else None)
     # executed FP64 MFMA flops per round (SQ_INSTS_VALU_MFMA_MOPS_F64 x 512 from a
     # counter pass) over the live busy time
     mc = (traffic or {}).get("mfma_counters")

@@ -348,7 +348,7 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
     // Re-derive totals from the planned host table and check the plan is
     // consistent (a table that was not planned, or edited, is rejected).
     int64_t need = kHeaderBytes, tiles = 0, pairs = 0, rows = 0, outs = 0;
-    int rows_nb = 0, gram_nb = 0, gram_y = 0, gram_c = 0, gram_nt = 0;
+    int rows_nb = 0, gram_nb = 0, gram_y = 0, gram_c = 0;
     bool has_d[5] = {false, false, false, false, false};  // index 1: plain splits
     bool has_b[17] = {false};  // one-sided problems by block width
     bool has_gram = false;
@@ -378,7 +378,6 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
             has_gram = true;
             gram_nb = std::max(gram_nb, p.nb);
             gram_y = std::max(gram_y, p.nb / 4 + (p.Cp + 63) / 64);
-            gram_nt = std::max(gram_nt, std::max(p.nb / 4, 1));
             gram_c = std::max(gram_c, (p.R + 31) / 32);
         }
     }
@@ -435,7 +434,7 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
                         "k_hsolve");
                 for (int round = 0; round < gram_nb - 2; ++round) {
                     HIP_TRY(timed("k_hupdate", stream,
-                                  [&] { return launch_hupdate(stream, d, owner, ws, pairs, gram_nt, round, 1); }),
+                                  [&] { return launch_hupdate(stream, d, owner, ws, pairs, 1, round, 1); }),
                             "k_hupdate");
                     HIP_TRY(timed("k_hround", stream,
                                   [&] {

@@ -917,6 +917,11 @@ MPSX_DEV int rr_partner(int item, int r, int n) {
     const int pos = item == 0 ? 0 : ((item - 1 - r) % (n - 1) + (n - 1)) % (n - 1) + 1;
     return rr_item(n - 1 - pos, r, n);
 }
+// Pivot (slot pair) that holds block `item` in round r.
+MPSX_DEV int rr_pivot(int item, int r, int n) {
+    const int pos = item == 0 ? 0 : ((item - 1 - r) % (n - 1) + (n - 1)) % (n - 1) + 1;
+    return pos < n - 1 - pos ? pos : n - 1 - pos;
+}
 
 // One workgroup per upper 16x16 block tile (bi <= bj) of H = X X^H. x: the
 // problem's pivot slot j (nb/2 per problem), y in [0, nb]; the tile is
@@ -1039,9 +1044,10 @@ k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, dou
 }
 
 // Round, part 2 (`part` 0: all of it; 1: only the tiles that hold a pivot
-// pair of round + 1, which its eigensolves read; 2: the other tiles and the
-// rows of Y, which may run beside the next round's eigensolves). x: pivot P
-// of a problem; y < n/2 (n = nb/2 pivots): the
+// pair of round + 1, which its eigensolves read, one workgroup per pivot of
+// round + 1 and y = 0; 2: the other tiles and the rows of Y, which may run
+// beside the next round's eigensolves). x: pivot P of a problem; y < n/2
+// (n = nb/2 pivots): the
 // 32x32 tile of H between P and P2 = (P + y + 1) mod n, each unordered pair
 // once, T <- Q_P^H T Q_P2 (a factor whose pivot did not rotate is the
 // identity and skipped); y >= n/2: columns 64 (y - n/2) .. +63 of the Y
@@ -1056,26 +1062,53 @@ MPSX_DEV void hupdate_body(const mpsx_two_site_op *ops, const int *owner, char *
     const mpsx_plan p = ops[pi].plan;
     if (p.method != 1 || round >= p.nb - 1) return;
     if (ctl_ptr(ws, p)->conv) return;
-    const int n = p.nb / 2, nt = n / 2;
-    const int P = (int)(blockIdx.x - p.pair_base);
+    const int n = p.nb / 2, nt = n / 2, nb = p.nb;
+    int P = (int)(blockIdx.x - p.pair_base);
     const int *flag = qflag_ptr(ws, p, round);
     const double2 *Qall = qmat_ptr(ws, p, round);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
-    int I, J;
-    rr_pair(P, round, p.nb, &I, &J);
-    if (y < nt) {
+    int P2 = -1;
+    if (part == 1) {
+        // one workgroup per pivot Pn of round + 1: the tile between the two
+        // pivots of this round that held its blocks (none when one pivot held
+        // both: its eigensolve has written that block), updated by the lowest
+        // pivot of round + 1 whose cross block lies in it
+        int In, Jn;
+        rr_pair(P, round + 1, nb, &In, &Jn);
+        const int pa = rr_pivot(In, round, nb), pb = rr_pivot(Jn, round, nb);
+        if (pa == pb) return;
+        int Ia, Ja, Ib, Jb;
+        rr_pair(pa, round, nb, &Ia, &Ja);
+        rr_pair(pb, round, nb, &Ib, &Jb);
+        int first = n;
+        for (int x : {Ia, Ja}) {
+            const int z = rr_partner(x, round + 1, nb);
+            if (z == Ib || z == Jb) first = min(first, rr_pivot(x, round + 1, nb));
+        }
+        if (first != P) return;
+        // the orientation the tile enumeration below uses (P2 = P + d mod n,
+        // 1 <= d <= n/2), so that a tile's arithmetic does not depend on
+        // which launch updates it
+        const int d = (pb - pa + n) % n;
+        const bool keep = 2 * d < n || (2 * d == n && pa < nt);
+        P = keep ? pa : pb;
+        P2 = keep ? pb : pa;
+    } else if (y < nt) {
         const int dd = y + 1;
         if (2 * dd == n && P >= nt) return;
-        const int P2 = (P + dd) % n;
+        P2 = (P + dd) % n;
+    }
+    int I, J;
+    rr_pair(P, round, nb, &I, &J);
+    if (P2 >= 0) {
         const int fa = flag[P], fb = flag[P2];
         if (!fa && !fb) return;
         int I2, J2;
-        rr_pair(P2, round, p.nb, &I2, &J2);
-        if (part != 0) {  // is one of the four block tiles a pivot pair of the next round?
-            const int pi_ = rr_partner(I, round + 1, p.nb), pj_ = rr_partner(J, round + 1, p.nb);
-            const bool crit = pi_ == I2 || pi_ == J2 || pj_ == I2 || pj_ == J2;
-            if (crit != (part == 1)) return;
+        rr_pair(P2, round, nb, &I2, &J2);
+        if (part == 2) {  // a tile holding a pivot pair of the next round was done by part 1
+            const int pi_ = rr_partner(I, round + 1, nb), pj_ = rr_partner(J, round + 1, nb);
+            if (pi_ == I2 || pi_ == J2 || pj_ == I2 || pj_ == J2) return;
         }
         double2 *Hg = h_ptr(ws, p);
         const int Cp = p.Cp;

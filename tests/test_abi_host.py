@@ -258,3 +258,30 @@ def test_gram_pipelined_round_split(nb):
                     have |= {(min(x, z), max(x, z)) for x in (I, J) for z in (I2, J2)}
         for I, J in nxt:
             assert {(I, I), (I, J), (J, J)} <= have, (nb, r, I, J)
+        # part 1 runs one workgroup per pivot of round r + 1; each critical tile
+        # (a pair of pivots of round r) must be claimed by exactly one of them
+        def pivot(item, rr):
+            pos = 0 if item == 0 else ((item - 1 - rr) % (nb - 1)) + 1
+            return min(pos, nb - 1 - pos)
+        critical = set()
+        for P in range(n):
+            for y in range(n // 2):
+                dd = y + 1
+                if 2 * dd == n and P >= n // 2:
+                    continue
+                I, J = piv[P]
+                I2, J2 = piv[(P + dd) % n]
+                if {_rr_partner(I, r + 1, nb), _rr_partner(J, r + 1, nb)} & {I2, J2}:
+                    critical.add(frozenset((P, (P + dd) % n)))
+        claimed = []
+        for Pn, (In, Jn) in enumerate(nxt):
+            pa, pb = pivot(In, r % (nb - 1)), pivot(Jn, r % (nb - 1))
+            if pa == pb:
+                continue
+            first = n
+            for x in piv[pa]:
+                if _rr_partner(x, r + 1, nb) in piv[pb]:
+                    first = min(first, pivot(x, (r + 1) % (nb - 1)))
+            if first == Pn:
+                claimed.append(frozenset((pa, pb)))
+        assert len(claimed) == len(set(claimed)) and set(claimed) == critical, (nb, r)
