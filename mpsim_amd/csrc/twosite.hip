@@ -993,11 +993,11 @@ MPSX_DEV void hsolve_body(const mpsx_two_site_op *ops, const int *owner, char *w
                           double2 (*H)[33], double2 (*Q)[33], double *prm_c, double2 *prm_s,
                           int &s_any) {
     constexpr int B2 = 32, b = 16, NT = HSOLVE_THREADS;
+    static_assert((B2 * B2) % NT == 0, "pivot block loads per thread");
     const int pi = owner[blockIdx.x];
     const mpsx_plan p = ops[pi].plan;
     if (p.method != 1 || round >= p.nb - 1) return;
     ProblemCtl *ctl = ctl_ptr(ws, p);
-    if (ctl->conv) return;
     const int pair = (int)(blockIdx.x - p.pair_base);
     int I, J;
     rr_pair(pair, round, p.nb, &I, &J);
@@ -1005,12 +1005,23 @@ MPSX_DEV void hsolve_body(const mpsx_two_site_op *ops, const int *owner, char *w
     double2 *Hg = h_ptr(ws, p);
     const int Cp = p.Cp, tid = threadIdx.x;
     // element (i, j), i <= j, lies in a stored tile: (I, I), (I, J) or (J, J);
-    // read row-contiguously, mirrored into the lower triangle
-    for (int e = tid; e < B2 * B2; e += NT) {
-        const int i = e / B2, j = e % B2;
+    // read row-contiguously (all loads in flight together with the convergence
+    // flag: the addresses are valid either way), mirrored into the lower
+    // triangle
+    constexpr int NE = B2 * B2 / NT;
+    double2 hv[NE];
+#pragma unroll
+    for (int t = 0; t < NE; ++t) {
+        const int e = tid + NT * t, i = e / B2, j = e % B2;
+        if (i <= j) hv[t] = Hg[(int64_t)grow(i) * Cp + grow(j)];
+    }
+    if (ctl->conv) return;
+#pragma unroll
+    for (int t = 0; t < NE; ++t) {
+        const int e = tid + NT * t, i = e / B2, j = e % B2;
         Q[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
         if (i > j) continue;
-        double2 h = Hg[(int64_t)grow(i) * Cp + grow(j)];
+        double2 h = hv[t];
         if (i == j) h.y = 0.0;
         H[i][j] = h;
         H[j][i] = make_double2(h.x, -h.y);
@@ -1061,7 +1072,7 @@ MPSX_DEV void hupdate_body(const mpsx_two_site_op *ops, const int *owner, char *
     const int pi = owner[blockIdx.x];
     const mpsx_plan p = ops[pi].plan;
     if (p.method != 1 || round >= p.nb - 1) return;
-    if (ctl_ptr(ws, p)->conv) return;
+    const int *convp = &ctl_ptr(ws, p)->conv;  // read with the first data loads below
     const int n = p.nb / 2, nt = n / 2, nb = p.nb;
     int P = (int)(blockIdx.x - p.pair_base);
     const int *flag = qflag_ptr(ws, p, round);
@@ -1102,8 +1113,6 @@ MPSX_DEV void hupdate_body(const mpsx_two_site_op *ops, const int *owner, char *
     int I, J;
     rr_pair(P, round, nb, &I, &J);
     if (P2 >= 0) {
-        const int fa = flag[P], fb = flag[P2];
-        if (!fa && !fb) return;
         int I2, J2;
         rr_pair(P2, round, nb, &I2, &J2);
         if (part == 2) {  // a tile holding a pivot pair of the next round was done by part 1
@@ -1115,20 +1124,36 @@ MPSX_DEV void hupdate_body(const mpsx_two_site_op *ops, const int *owner, char *
         // The tile's four 16x16 block tiles (row block a of P: I or J, column
         // block c of P2: I2 or J2) are stored as H[bi][bj] when bi < bj, else
         // as the conjugate transpose in H[bj][bi]; both are read and written
-        // row-contiguously (256 B per block row) and transposed in LDS.
+        // row-contiguously (256 B per block row) and transposed in LDS. The
+        // loads are issued together with the convergence and rotation flags
+        // (valid addresses either way).
         const int ii = tid >> 4, jj = tid & 15;  // one element of each block tile per thread
         int64_t gaddr[4];
         bool lower[4];
+        double2 tv[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int bi = (t >> 1) ? J : I, bj = (t & 1) ? J2 : I2;
             lower[t] = bi > bj;
             gaddr[t] = lower[t] ? (int64_t)(bj * b + ii) * Cp + bi * b + jj
                                 : (int64_t)(bi * b + ii) * Cp + bj * b + jj;
+            tv[t] = Hg[gaddr[t]];
         }
+        const int fa = flag[P], fb = flag[P2];
+        const int tm = wave >> 1, tn = wave & 1;
+        // this lane's rotation operands of both products, loaded up front
+        const double2 *Qa = Qall + (int64_t)P * 1024, *Qb = Qall + (int64_t)P2 * 1024;
+        double2 qa[8], qb[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int k = 4 * s + lq;
+            qa[s] = Qa[k * 32 + 16 * tm + l16];  // conj: A = Q_P^H[16 tm + l16][k]
+            qb[s] = Qb[k * 32 + 16 * tn + l16];  // B = Q_P2[k][16 tn + l16]
+        }
+        if (*convp || (!fa && !fb)) return;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const double2 v = Hg[gaddr[t]];
+            const double2 v = tv[t];
             const int r0 = 16 * (t >> 1), c0 = 16 * (t & 1);
             if (lower[t])
                 Ts[r0 + jj][c0 + ii] = make_double2(v.x, -v.y);
@@ -1136,16 +1161,14 @@ MPSX_DEV void hupdate_body(const mpsx_two_site_op *ops, const int *owner, char *
                 Ts[r0 + ii][c0 + jj] = v;
         }
         __syncthreads();
-        const int tm = wave >> 1, tn = wave & 1;
         // W = Q_P^H T: register r of lane (l16, lq) is W[16 tm + 4 r + lq][16 tn + l16]
         if (fa) {
-            const double2 *Qa = Qall + (int64_t)P * 1024;
             mfma_d4 a1 = {0.0, 0.0, 0.0, 0.0}, a2 = a1, a3 = a1;
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
                 const int k = 4 * s + lq;
-                const double2 q = Qa[k * 32 + 16 * tm + l16];  // conj: A = Q^H[16 tm + l16][k]
-                const double2 t = Ts[k][16 * tn + l16];        // B = T[k][16 tn + l16]
+                const double2 q = qa[s];                   // conj: A = Q^H[16 tm + l16][k]
+                const double2 t = Ts[k][16 * tn + l16];    // B = T[k][16 tn + l16]
                 a1 = mfma16d(q.x, t.x, a1);
                 a2 = mfma16d(q.y, t.y, a2);
                 a3 = mfma16d(q.x - q.y, t.x + t.y, a3);
@@ -1158,16 +1181,15 @@ MPSX_DEV void hupdate_body(const mpsx_two_site_op *ops, const int *owner, char *
         }
         if (fb) {
             const double2(*Wsrc)[33] = fa ? Ws : Ts;
-            const double2 *Qb = Qall + (int64_t)P2 * 1024;
             mfma_d4 c1 = {0.0, 0.0, 0.0, 0.0}, c2 = c1, c3 = c1;
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
                 const int k = 4 * s + lq;
-                const double2 wa = Wsrc[16 * tm + l16][k];         // A = W[16 tm + l16][k]
-                const double2 qb = Qb[k * 32 + 16 * tn + l16];     // B = Q_P2[k][16 tn + l16]
-                c1 = mfma16d(wa.x, qb.x, c1);
-                c2 = mfma16d(wa.y, qb.y, c2);
-                c3 = mfma16d(wa.x + wa.y, qb.x + qb.y, c3);
+                const double2 wa = Wsrc[16 * tm + l16][k];  // A = W[16 tm + l16][k]
+                const double2 qv = qb[s];                   // B = Q_P2[k][16 tn + l16]
+                c1 = mfma16d(wa.x, qv.x, c1);
+                c2 = mfma16d(wa.y, qv.y, c2);
+                c3 = mfma16d(wa.x + wa.y, qv.x + qv.y, c3);
             }
             __syncthreads();  // every wave has read Ts (as W) before it is overwritten
 #pragma unroll
@@ -1192,7 +1214,7 @@ MPSX_DEV void hupdate_body(const mpsx_two_site_op *ops, const int *owner, char *
     // Y rows of pivot P, 16 columns per wave
     if (part == 1) return;
     const int c0 = (y - nt) * 64;
-    if (c0 >= p.Cp || !flag[P]) return;
+    if (c0 >= p.Cp || *convp || !flag[P]) return;
     const double2 *Qa = Qall + (int64_t)P * 1024;
     double2 *Z = z_ptr<double>(ws, p);
     const int LD = p.LD, R = p.R;
