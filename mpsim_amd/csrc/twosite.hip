@@ -889,11 +889,15 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
 // so it reads the pivot rows of [X | Y] twice and writes them once per
 // round; here H is formed once, a round reads and writes the upper block
 // tiles of H and the rows of Y, and X is rotated once at the end,
-// Xt = Y X. A round is two launches: k_hsolve (one workgroup per pivot:
-// convergence test and eigensolve of the pivot block of H, latency-bound)
-// and k_hupdate (one workgroup per pair of pivots and per 64 columns of Y:
-// matrix-core products, memory-bound), so the eigensolves of one call can
-// overlap the updates of another call running on another stream.
+// Xt = Y X. A round has two parts: the eigensolves (one workgroup per
+// pivot: convergence test and eigensolve of the pivot block of H,
+// latency-bound) and the update (one workgroup per pair of pivots and per
+// 64 columns of Y: matrix-core products, memory-bound). The update is split
+// once more: its part 1 holds the tiles the next round's eigensolves read;
+// part 2, the bulk, runs in the same launch as those eigensolves
+// (k_hround), so the memory-bound work of a round overlaps the latency of
+// the next. A sweep is k_hsolve(0), then k_hupdate(r, part 1) and
+// k_hround(r) for r = 0 .. R-2, then k_hupdate(R-1, whole).
 //
 // H is stored as Cp x Cp (row stride Cp) with the 16x16 block tiles
 // (bi <= bj) valid; of the diagonal block tiles only the upper triangles are
@@ -1054,15 +1058,14 @@ k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, dou
     hsolve_body(ops, owner, ws, round, tol, tol_inner, inner_sweeps, abs_rel, H, Q, prm_c, prm_s, s_any);
 }
 
-// Round, part 2 (`part` 0: all of it; 1: only the tiles that hold a pivot
-// pair of round + 1, which its eigensolves read, one workgroup per pivot of
-// round + 1 and y = 0; 2: the other tiles and the rows of Y, which may run
-// beside the next round's eigensolves). x: pivot P of a problem; y < n/2
-// (n = nb/2 pivots): the
-// 32x32 tile of H between P and P2 = (P + y + 1) mod n, each unordered pair
-// once, T <- Q_P^H T Q_P2 (a factor whose pivot did not rotate is the
-// identity and skipped); y >= n/2: columns 64 (y - n/2) .. +63 of the Y
-// rows of P, Y <- Q_P^H Y. FP64 16x16x4 MFMAs, three real products per
+// Round, part 2: the update (`part` 0: all of it; 1: only the tiles that
+// hold a pivot pair of round + 1, which its eigensolves read, with one
+// workgroup per pivot of round + 1 and y = 0; 2: the other tiles and the
+// rows of Y, which may run beside the next round's eigensolves). x: pivot P
+// of a problem; y < n/2 (n = nb/2 pivots): the 32x32 tile of H between P
+// and P2 = (P + y + 1) mod n, each unordered pair once, T <- Q_P^H T Q_P2 (a
+// factor whose pivot did not rotate is the identity and skipped); y >= n/2:
+// columns 64 (y - n/2) .. +63 of the Y rows of P, Y <- Q_P^H Y. FP64 16x16x4 MFMAs, three real products per
 // complex product; wave w owns output tile (w / 2, w % 2) of the H tile, or
 // 16 columns of all 32 Y rows. Tiles pass through LDS (Ts: T, then the
 // result; Ws: W = Q_P^H T).
