@@ -59,3 +59,49 @@ def test_fp32_rotation_mode_runs():
     err, err_c64 = _overlap_errors(circ, SvdConfig(block_switch=16, fp32=True))
     print(f"fp32 rotations: overlap error {err:.3e} (complex64 LAPACK {err_c64:.3e})")
     assert err <= 1e-2
+
+
+def test_gram_and_row_methods_agree():
+    """The Gram method (two-sided Jacobi on H = X X^H, the default for FP64
+    problems with 32-row pivots) and the one-sided row iteration
+    (SvdConfig(method=1)) on the same batch, mixed sizes (16-row pivots, 32-row
+    pivots with 2 .. 32 blocks, absorb left and right): the truncated two-site
+    products and the kept singular values agree to the precision of their
+    complex64 / float32 outputs (1e-6 of ||theta||, of sigma_max), the FP64
+    discarded weights to 1e-10 of ||theta||^2, and both match LAPACK on theta
+    in FP64."""
+    import torch
+    from mpsim_amd import gates
+    from mpsim_amd.engine import Engine, SvdConfig, TwoSiteItem
+    eng = Engine.get()
+    rng = np.random.RandomState(23)
+    shapes = [(4, 4, 4, 8, False), (16, 12, 16, 20, True), (64, 64, 64, 64, False),
+              (128, 96, 128, 200, True), (256, 256, 256, 256, False), (256, 128, 64, 100, False)]
+    items, thetas = [], []
+    for cl, cm, cr, k, left in shapes:
+        A = ((rng.randn(cl, 2, cm) + 1j * rng.randn(cl, 2, cm)) / np.sqrt(4 * cm)).astype(np.complex64)
+        B = ((rng.randn(cm, 2, cr) + 1j * rng.randn(cm, 2, cr)) / np.sqrt(2)).astype(np.complex64)
+        G = gates.haar_unitary(4, rng)
+        items.append(TwoSiteItem(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), eng.gate(G),
+                                 k, absorb_left=left, want_s=True))
+        th = np.einsum("abpq,lpqr->labr", G.reshape(2, 2, 2, 2).astype(np.complex128),
+                       np.tensordot(A.astype(np.complex128), B.astype(np.complex128), axes=([2], [0])))
+        thetas.append(th.reshape(2 * cl, 2 * cr))
+    out = {}
+    for method in (0, 1):
+        res = eng.two_site(items, SvdConfig(block_switch=16, method=method))
+        torch.cuda.synchronize()
+        out[method] = [(r.a_out.cpu().numpy().astype(np.complex128),
+                        r.b_out.cpu().numpy().astype(np.complex128),
+                        r.s.cpu().numpy().astype(np.float64), float(r.discarded.cpu())) for r in res]
+    for (cl, cm, cr, k, left), th, g, o in zip(shapes, thetas, out[0], out[1]):
+        M, N = th.shape
+        rec_g = g[0].reshape(M, k) @ g[1].reshape(k, N)
+        rec_o = o[0].reshape(M, k) @ o[1].reshape(k, N)
+        scale = np.linalg.norm(th)
+        assert np.linalg.norm(rec_g - rec_o) <= 1e-6 * scale, (cl, cm, cr, k)
+        s_ref = np.linalg.svd(th, compute_uv=False)
+        assert np.max(np.abs(g[2] - o[2])) <= 1e-6 * s_ref[0]
+        assert np.max(np.abs(g[2] - s_ref[:k])) <= 1e-6 * s_ref[0]
+        assert abs(g[3] - o[3]) <= 1e-10 * scale ** 2 + 1e-6 * abs(o[3])
+        assert abs(g[3] - np.sum(s_ref[k:] ** 2)) <= 1e-6 * scale ** 2
