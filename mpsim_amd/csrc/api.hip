@@ -150,10 +150,12 @@ int64_t theta_tiles(const mpsx_two_site_op &o) {
 // crossover; 64 by default).
 int auto_block(int C, int switch_rows) { return C <= switch_rows ? 8 : 16; }
 
-// The Gram method serves FP64 problems with 32-row pivots unless the
-// one-sided method is asked for.
-int plan_method(const Resolved &r, int b) {
-    return (!r.fp32 && b == 16 && r.method != MPSX_SVD_ROWS) ? 1 : 0;
+// The Gram method serves FP64 problems with 32-row pivots and at least six
+// 16-row blocks unless the one-sided method is asked for (with four blocks
+// one H tile holds the cross blocks of both pivots of the next round, which
+// the pipelined rounds do not allow).
+int plan_method(const Resolved &r, int b, int nb) {
+    return (!r.fp32 && b == 16 && nb >= 6 && r.method != MPSX_SVD_ROWS) ? 1 : 0;
 }
 
 int *poll_buffer() {
@@ -312,7 +314,7 @@ int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts
         }
         p.LD = (int32_t)align_up((int64_t)p.R + p.Cp, 4);
         p.zbytes = r.zbytes;
-        p.method = p.nb > 0 ? plan_method(r, p.b) : 0;
+        p.method = p.nb > 0 ? plan_method(r, p.b, p.nb) : 0;
         p.ws_off = off;
         p.tile_base = tiles;
         p.pair_base = pairs;
@@ -362,7 +364,7 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
             return fail(MPSX_EINVAL, "op %d: invalid plan", i);
         if (p.zbytes != r.zbytes)
             return fail(MPSX_EINVAL, "op %d: planned for the other rotation precision", i);
-        if (p.method != (p.nb > 0 ? plan_method(r, p.b) : 0))
+        if (p.method != (p.nb > 0 ? plan_method(r, p.b, p.nb) : 0))
             return fail(MPSX_EINVAL, "op %d: planned for another SVD method", i);
         need += problem_bytes(p);
         tiles += theta_tiles(o);
@@ -421,9 +423,9 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
                                       }),
                                 "k_jacobi_round");
             // Gram rounds 0 .. R-1 (R = gram_nb - 1), pipelined on the stream:
-            //   k_hsolve(0); then for r < R-1: k_hupdate(r, tiles the
-            //   eigensolves of r+1 read), k_hround(eigensolves of r+1 beside
-            //   the rest of round r's update); finally k_hupdate(R-1, whole).
+            //   k_hsolve(0); then for r < R-1: k_hround(eigensolves of r+1,
+            //   each after the tile holding its cross block, beside the rest
+            //   of round r's update); finally k_hupdate(R-1, whole).
             if (has_gram) {
                 const double ti = r.tol * 0.01;
                 HIP_TRY(timed("k_hsolve", stream,
@@ -433,9 +435,6 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
                               }),
                         "k_hsolve");
                 for (int round = 0; round < gram_nb - 2; ++round) {
-                    HIP_TRY(timed("k_hupdate", stream,
-                                  [&] { return launch_hupdate(stream, d, owner, ws, pairs, 1, round, 1); }),
-                            "k_hupdate");
                     HIP_TRY(timed("k_hround", stream,
                                   [&] {
                                       return launch_hround(stream, d, owner, ws, pairs, gram_y, round, r.tol, ti,

@@ -892,12 +892,13 @@ k_jacobi_round64(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
 // Xt = Y X. A round has two parts: the eigensolves (one workgroup per
 // pivot: convergence test and eigensolve of the pivot block of H,
 // latency-bound) and the update (one workgroup per pair of pivots and per
-// 64 columns of Y: matrix-core products, memory-bound). The update is split
-// once more: its part 1 holds the tiles the next round's eigensolves read;
-// part 2, the bulk, runs in the same launch as those eigensolves
-// (k_hround), so the memory-bound work of a round overlaps the latency of
-// the next. A sweep is k_hsolve(0), then k_hupdate(r, part 1) and
-// k_hround(r) for r = 0 .. R-2, then k_hupdate(R-1, whole).
+// 64 columns of Y: matrix-core products, memory-bound). Rounds are
+// pipelined: the eigensolve workgroup of a pivot of round r + 1 first
+// updates (round r) the one tile that holds its cross block, and the rest of
+// round r's update runs in the same launch (k_hround), so the memory-bound
+// work of a round overlaps the latency of the next and the chain of a sweep
+// is one launch per round. A sweep is k_hsolve(0), then k_hround(r) for
+// r = 0 .. R-2, then k_hupdate(R-1, whole).
 //
 // H is stored as Cp x Cp (row stride Cp) with the 16x16 block tiles
 // (bi <= bj) valid; of the diagonal block tiles only the upper triangles are
@@ -984,42 +985,40 @@ k_hgram(const mpsx_two_site_op *ops, const int *owner, char *ws) {
 // four: the per-step cost is latency, not issue).
 constexpr int HSOLVE_THREADS = 256;
 
-// Round, part 1: one workgroup per pivot (blocks I < J of round `round`).
-// Loads the pivot's 32x32 block of H, runs the convergence test and the
-// pivot eigensolve, and writes Q and the rotated block (Q^H H_PP Q, which
-// the eigensolve leaves in LDS) back; flag[P] tells part 2 whether Q is
-// the identity. The absolute floor abs_rel * ||theta||_F^2 sits above the
-// FP64 rounding H gathers over the rounds (pairs below it would otherwise
-// keep rotating on noise; their residual coupling changes the truncated
-// factors only to second order).
-MPSX_DEV void hsolve_body(const mpsx_two_site_op *ops, const int *owner, char *ws, int round,
-                          double tol, double tol_inner, int inner_sweeps, double abs_rel,
-                          double2 (*H)[33], double2 (*Q)[33], double *prm_c, double2 *prm_s,
-                          int &s_any) {
-    constexpr int B2 = 32, b = 16, NT = HSOLVE_THREADS;
-    static_assert((B2 * B2) % NT == 0, "pivot block loads per thread");
-    const int pi = owner[blockIdx.x];
-    const mpsx_plan p = ops[pi].plan;
-    if (p.method != 1 || round >= p.nb - 1) return;
+constexpr int HSOLVE_NE = 32 * 32 / HSOLVE_THREADS;  // pivot block elements per thread
+static_assert((32 * 32) % HSOLVE_THREADS == 0, "pivot block loads per thread");
+
+// This thread's elements (i, j), i <= j, of the 32x32 pivot block of blocks
+// I < J: element e = tid + 256 t of the block in row-major order lies in a
+// stored tile, (I, I), (I, J) or (J, J), read row-contiguously.
+MPSX_DEV void hsolve_load(const double2 *Hg, int Cp, int I, int J, double2 (&hv)[HSOLVE_NE]) {
+    constexpr int B2 = 32, b = 16;
+    auto grow = [&](int i) { return i < b ? I * b + i : J * b + (i - b); };
+#pragma unroll
+    for (int t = 0; t < HSOLVE_NE; ++t) {
+        const int e = threadIdx.x + HSOLVE_THREADS * t, i = e / B2, j = e % B2;
+        if (i <= j) hv[t] = Hg[(int64_t)grow(i) * Cp + grow(j)];
+    }
+}
+
+// The eigensolve of pivot `pair` (blocks I < J) of round `round` from the
+// loaded upper triangle hv of its 32x32 block of H: runs the pivot
+// eigensolve and writes Q and the rotated block (Q^H H_PP Q, which the
+// eigensolve leaves in LDS) back; the pivot's flag tells the update whether
+// Q is the identity. The absolute floor abs_rel * ||theta||_F^2 sits above
+// the FP64 rounding H gathers over the rounds (pairs below it would
+// otherwise keep rotating on noise; their residual coupling changes the
+// truncated factors only to second order).
+MPSX_DEV void hsolve_core(const mpsx_plan &p, char *ws, int round, int pair, int I, int J,
+                          const double2 (&hv)[HSOLVE_NE], double tol, double tol_inner, int inner_sweeps,
+                          double abs_rel, double2 (*H)[33], double2 (*Q)[33], double *prm_c,
+                          double2 *prm_s, int &s_any) {
+    constexpr int B2 = 32, b = 16, NT = HSOLVE_THREADS, NE = HSOLVE_NE;
     ProblemCtl *ctl = ctl_ptr(ws, p);
-    const int pair = (int)(blockIdx.x - p.pair_base);
-    int I, J;
-    rr_pair(pair, round, p.nb, &I, &J);
     auto grow = [&](int i) { return i < b ? I * b + i : J * b + (i - b); };
     double2 *Hg = h_ptr(ws, p);
     const int Cp = p.Cp, tid = threadIdx.x;
-    // element (i, j), i <= j, lies in a stored tile: (I, I), (I, J) or (J, J);
-    // read row-contiguously (all loads in flight together with the convergence
-    // flag: the addresses are valid either way), mirrored into the lower
-    // triangle
-    constexpr int NE = B2 * B2 / NT;
-    double2 hv[NE];
-#pragma unroll
-    for (int t = 0; t < NE; ++t) {
-        const int e = tid + NT * t, i = e / B2, j = e % B2;
-        if (i <= j) hv[t] = Hg[(int64_t)grow(i) * Cp + grow(j)];
-    }
-    if (ctl->conv) return;
+    // mirrored into the lower triangle
 #pragma unroll
     for (int t = 0; t < NE; ++t) {
         const int e = tid + NT * t, i = e / B2, j = e % B2;
@@ -1047,6 +1046,26 @@ MPSX_DEV void hsolve_body(const mpsx_two_site_op *ops, const int *owner, char *w
     if (tid == 0) flag[pair] = 1;
 }
 
+// Round `round`, eigensolves: one workgroup per pivot. Loads the pivot's
+// 32x32 block of H (all loads in flight together with the convergence flag:
+// the addresses are valid either way) and solves it.
+MPSX_DEV void hsolve_body(const mpsx_two_site_op *ops, const int *owner, char *ws, int round,
+                          double tol, double tol_inner, int inner_sweeps, double abs_rel,
+                          double2 (*H)[33], double2 (*Q)[33], double *prm_c, double2 *prm_s,
+                          int &s_any) {
+    const int pi = owner[blockIdx.x];
+    const mpsx_plan p = ops[pi].plan;
+    if (p.method != 1 || round >= p.nb - 1) return;
+    const int pair = (int)(blockIdx.x - p.pair_base);
+    int I, J;
+    rr_pair(pair, round, p.nb, &I, &J);
+    double2 hv[HSOLVE_NE];
+    hsolve_load(h_ptr(ws, p), p.Cp, I, J, hv);
+    if (ctl_ptr(ws, p)->conv) return;
+    hsolve_core(p, ws, round, pair, I, J, hv, tol, tol_inner, inner_sweeps, abs_rel, H, Q, prm_c, prm_s,
+                s_any);
+}
+
 __global__ void __launch_bounds__(HSOLVE_THREADS)
 k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
          double tol_inner, int inner_sweeps, double abs_rel) {
@@ -1058,167 +1077,158 @@ k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, dou
     hsolve_body(ops, owner, ws, round, tol, tol_inner, inner_sweeps, abs_rel, H, Q, prm_c, prm_s, s_any);
 }
 
-// Round, part 2: the update (`part` 0: all of it; 1: only the tiles that
-// hold a pivot pair of round + 1, which its eigensolves read, with one
-// workgroup per pivot of round + 1 and y = 0; 2: the other tiles and the
-// rows of Y, which may run beside the next round's eigensolves). x: pivot P
-// of a problem; y < n/2 (n = nb/2 pivots): the 32x32 tile of H between P
-// and P2 = (P + y + 1) mod n, each unordered pair once, T <- Q_P^H T Q_P2 (a
-// factor whose pivot did not rotate is the identity and skipped); y >= n/2:
-// columns 64 (y - n/2) .. +63 of the Y rows of P, Y <- Q_P^H Y. FP64 16x16x4 MFMAs, three real products per
-// complex product; wave w owns output tile (w / 2, w % 2) of the H tile, or
-// 16 columns of all 32 Y rows. Tiles pass through LDS (Ts: T, then the
-// result; Ws: W = Q_P^H T).
+// Result of an H tile update: the LDS array holding it.
+using HTile = double2[33];
+
+// T <- Q_P^H T Q_P2 in place for the 32x32 tile of H between pivots P != P2
+// of round `round` (row blocks I, J of P; column blocks I2, J2 of P2; a
+// factor whose pivot did not rotate is the identity and skipped). FP64
+// 16x16x4 MFMAs, three real products per complex product; wave w owns output
+// tile (w / 2, w % 2). The tile passes through LDS (Ts: T, then the result;
+// Ws: W = Q_P^H T). Returns the array holding the result, or nullptr when
+// nothing was written (problem converged, or neither pivot rotated). Uniform
+// over the workgroup.
+MPSX_DEV const HTile *htile_update(const mpsx_plan &p, char *ws, int round, int P, int P2,
+                                   double2 (*Ts)[33], double2 (*Ws)[33]) {
+    constexpr int b = 16;
+    const int *convp = &ctl_ptr(ws, p)->conv;  // read with the data loads below
+    const int *flag = qflag_ptr(ws, p, round);
+    const double2 *Qall = qmat_ptr(ws, p, round);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    int I, J, I2, J2;
+    rr_pair(P, round, p.nb, &I, &J);
+    rr_pair(P2, round, p.nb, &I2, &J2);
+    double2 *Hg = h_ptr(ws, p);
+    const int Cp = p.Cp;
+    // The tile's four 16x16 block tiles (row block a of P: I or J, column
+    // block c of P2: I2 or J2) are stored as H[bi][bj] when bi < bj, else
+    // as the conjugate transpose in H[bj][bi]; both are read and written
+    // row-contiguously (256 B per block row) and transposed in LDS. The
+    // loads are issued together with the convergence and rotation flags
+    // (valid addresses either way).
+    const int ii = tid >> 4, jj = tid & 15;  // one element of each block tile per thread
+    int64_t gaddr[4];
+    bool lower[4];
+    double2 tv[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int bi = (t >> 1) ? J : I, bj = (t & 1) ? J2 : I2;
+        lower[t] = bi > bj;
+        gaddr[t] = lower[t] ? (int64_t)(bj * b + ii) * Cp + bi * b + jj
+                            : (int64_t)(bi * b + ii) * Cp + bj * b + jj;
+        tv[t] = Hg[gaddr[t]];
+    }
+    const int fa = flag[P], fb = flag[P2];
+    const int tm = wave >> 1, tn = wave & 1;
+    // this lane's rotation operands of both products, loaded up front
+    const double2 *Qa = Qall + (int64_t)P * 1024, *Qb = Qall + (int64_t)P2 * 1024;
+    double2 qa[8], qb[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int k = 4 * s + lq;
+        qa[s] = Qa[k * 32 + 16 * tm + l16];  // conj: A = Q_P^H[16 tm + l16][k]
+        qb[s] = Qb[k * 32 + 16 * tn + l16];  // B = Q_P2[k][16 tn + l16]
+    }
+    if (*convp || (!fa && !fb)) return nullptr;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const double2 v = tv[t];
+        const int r0 = 16 * (t >> 1), c0 = 16 * (t & 1);
+        if (lower[t])
+            Ts[r0 + jj][c0 + ii] = make_double2(v.x, -v.y);
+        else
+            Ts[r0 + ii][c0 + jj] = v;
+    }
+    __syncthreads();
+    // W = Q_P^H T: register r of lane (l16, lq) is W[16 tm + 4 r + lq][16 tn + l16]
+    if (fa) {
+        mfma_d4 a1 = {0.0, 0.0, 0.0, 0.0}, a2 = a1, a3 = a1;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int k = 4 * s + lq;
+            const double2 q = qa[s];                   // conj: A = Q^H[16 tm + l16][k]
+            const double2 t = Ts[k][16 * tn + l16];    // B = T[k][16 tn + l16]
+            a1 = mfma16d(q.x, t.x, a1);
+            a2 = mfma16d(q.y, t.y, a2);
+            a3 = mfma16d(q.x - q.y, t.x + t.y, a3);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            Ws[16 * tm + 4 * r + lq][16 * tn + l16] =
+                make_double2(a1[r] + a2[r], a3[r] - a1[r] + a2[r]);
+        __syncthreads();
+    }
+    if (fb) {
+        const double2(*Wsrc)[33] = fa ? Ws : Ts;
+        mfma_d4 c1 = {0.0, 0.0, 0.0, 0.0}, c2 = c1, c3 = c1;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int k = 4 * s + lq;
+            const double2 wa = Wsrc[16 * tm + l16][k];  // A = W[16 tm + l16][k]
+            const double2 qv = qb[s];                   // B = Q_P2[k][16 tn + l16]
+            c1 = mfma16d(wa.x, qv.x, c1);
+            c2 = mfma16d(wa.y, qv.y, c2);
+            c3 = mfma16d(wa.x + wa.y, qv.x + qv.y, c3);
+        }
+        __syncthreads();  // every wave has read Ts (as W) before it is overwritten
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            Ts[16 * tm + 4 * r + lq][16 * tn + l16] =
+                make_double2(c1[r] - c2[r], c3[r] - c1[r] - c2[r]);
+        __syncthreads();
+    }
+    const HTile *Out = fb ? Ts : Ws;  // fa only: the result is W
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int r0 = 16 * (t >> 1), c0 = 16 * (t & 1);
+        if (lower[t]) {
+            const double2 v = Out[r0 + jj][c0 + ii];
+            Hg[gaddr[t]] = make_double2(v.x, -v.y);
+        } else {
+            Hg[gaddr[t]] = Out[r0 + ii][c0 + jj];
+        }
+    }
+    return Out;
+}
+
+// Round `round`, update (`part` 0: all of it; 2: all but the tiles that hold
+// the cross block of a pivot of round + 1, which the workgroups of round
+// + 1's eigensolves update themselves, see hnext_body). x: pivot P of a
+// problem; y < n/2 (n = nb/2 pivots): the 32x32 tile of H between P and
+// P2 = (P + y + 1) mod n, each unordered pair once; y >= n/2: columns
+// 64 (y - n/2) .. +63 of the Y rows of P, Y <- Q_P^H Y (FP64 MFMAs, wave w
+// owns 16 columns of all 32 rows).
 MPSX_DEV void hupdate_body(const mpsx_two_site_op *ops, const int *owner, char *ws, int round,
                            int part, int y, double2 (*Ts)[33], double2 (*Ws)[33]) {
     constexpr int b = 16;
     const int pi = owner[blockIdx.x];
     const mpsx_plan p = ops[pi].plan;
     if (p.method != 1 || round >= p.nb - 1) return;
-    const int *convp = &ctl_ptr(ws, p)->conv;  // read with the first data loads below
     const int n = p.nb / 2, nt = n / 2, nb = p.nb;
-    int P = (int)(blockIdx.x - p.pair_base);
-    const int *flag = qflag_ptr(ws, p, round);
-    const double2 *Qall = qmat_ptr(ws, p, round);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int l16 = lane & 15, lq = lane >> 4;
-    int P2 = -1;
-    if (part == 1) {
-        // one workgroup per pivot Pn of round + 1: the tile between the two
-        // pivots of this round that held its blocks (none when one pivot held
-        // both: its eigensolve has written that block), updated by the lowest
-        // pivot of round + 1 whose cross block lies in it
-        int In, Jn;
-        rr_pair(P, round + 1, nb, &In, &Jn);
-        const int pa = rr_pivot(In, round, nb), pb = rr_pivot(Jn, round, nb);
-        if (pa == pb) return;
-        int Ia, Ja, Ib, Jb;
-        rr_pair(pa, round, nb, &Ia, &Ja);
-        rr_pair(pb, round, nb, &Ib, &Jb);
-        int first = n;
-        for (int x : {Ia, Ja}) {
-            const int z = rr_partner(x, round + 1, nb);
-            if (z == Ib || z == Jb) first = min(first, rr_pivot(x, round + 1, nb));
-        }
-        if (first != P) return;
-        // the orientation the tile enumeration below uses (P2 = P + d mod n,
-        // 1 <= d <= n/2), so that a tile's arithmetic does not depend on
-        // which launch updates it
-        const int d = (pb - pa + n) % n;
-        const bool keep = 2 * d < n || (2 * d == n && pa < nt);
-        P = keep ? pa : pb;
-        P2 = keep ? pb : pa;
-    } else if (y < nt) {
-        const int dd = y + 1;
-        if (2 * dd == n && P >= nt) return;
-        P2 = (P + dd) % n;
-    }
+    const int P = (int)(blockIdx.x - p.pair_base);
     int I, J;
     rr_pair(P, round, nb, &I, &J);
-    if (P2 >= 0) {
-        int I2, J2;
-        rr_pair(P2, round, nb, &I2, &J2);
-        if (part == 2) {  // a tile holding a pivot pair of the next round was done by part 1
+    if (y < nt) {
+        const int dd = y + 1;
+        if (2 * dd == n && P >= nt) return;
+        const int P2 = (P + dd) % n;
+        if (part == 2) {  // a tile holding a pivot pair of the next round is updated with its eigensolve
+            int I2, J2;
+            rr_pair(P2, round, nb, &I2, &J2);
             const int pi_ = rr_partner(I, round + 1, nb), pj_ = rr_partner(J, round + 1, nb);
             if (pi_ == I2 || pi_ == J2 || pj_ == I2 || pj_ == J2) return;
         }
-        double2 *Hg = h_ptr(ws, p);
-        const int Cp = p.Cp;
-        // The tile's four 16x16 block tiles (row block a of P: I or J, column
-        // block c of P2: I2 or J2) are stored as H[bi][bj] when bi < bj, else
-        // as the conjugate transpose in H[bj][bi]; both are read and written
-        // row-contiguously (256 B per block row) and transposed in LDS. The
-        // loads are issued together with the convergence and rotation flags
-        // (valid addresses either way).
-        const int ii = tid >> 4, jj = tid & 15;  // one element of each block tile per thread
-        int64_t gaddr[4];
-        bool lower[4];
-        double2 tv[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int bi = (t >> 1) ? J : I, bj = (t & 1) ? J2 : I2;
-            lower[t] = bi > bj;
-            gaddr[t] = lower[t] ? (int64_t)(bj * b + ii) * Cp + bi * b + jj
-                                : (int64_t)(bi * b + ii) * Cp + bj * b + jj;
-            tv[t] = Hg[gaddr[t]];
-        }
-        const int fa = flag[P], fb = flag[P2];
-        const int tm = wave >> 1, tn = wave & 1;
-        // this lane's rotation operands of both products, loaded up front
-        const double2 *Qa = Qall + (int64_t)P * 1024, *Qb = Qall + (int64_t)P2 * 1024;
-        double2 qa[8], qb[8];
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const int k = 4 * s + lq;
-            qa[s] = Qa[k * 32 + 16 * tm + l16];  // conj: A = Q_P^H[16 tm + l16][k]
-            qb[s] = Qb[k * 32 + 16 * tn + l16];  // B = Q_P2[k][16 tn + l16]
-        }
-        if (*convp || (!fa && !fb)) return;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const double2 v = tv[t];
-            const int r0 = 16 * (t >> 1), c0 = 16 * (t & 1);
-            if (lower[t])
-                Ts[r0 + jj][c0 + ii] = make_double2(v.x, -v.y);
-            else
-                Ts[r0 + ii][c0 + jj] = v;
-        }
-        __syncthreads();
-        // W = Q_P^H T: register r of lane (l16, lq) is W[16 tm + 4 r + lq][16 tn + l16]
-        if (fa) {
-            mfma_d4 a1 = {0.0, 0.0, 0.0, 0.0}, a2 = a1, a3 = a1;
-#pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                const int k = 4 * s + lq;
-                const double2 q = qa[s];                   // conj: A = Q^H[16 tm + l16][k]
-                const double2 t = Ts[k][16 * tn + l16];    // B = T[k][16 tn + l16]
-                a1 = mfma16d(q.x, t.x, a1);
-                a2 = mfma16d(q.y, t.y, a2);
-                a3 = mfma16d(q.x - q.y, t.x + t.y, a3);
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                Ws[16 * tm + 4 * r + lq][16 * tn + l16] =
-                    make_double2(a1[r] + a2[r], a3[r] - a1[r] + a2[r]);
-            __syncthreads();
-        }
-        if (fb) {
-            const double2(*Wsrc)[33] = fa ? Ws : Ts;
-            mfma_d4 c1 = {0.0, 0.0, 0.0, 0.0}, c2 = c1, c3 = c1;
-#pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                const int k = 4 * s + lq;
-                const double2 wa = Wsrc[16 * tm + l16][k];  // A = W[16 tm + l16][k]
-                const double2 qv = qb[s];                   // B = Q_P2[k][16 tn + l16]
-                c1 = mfma16d(wa.x, qv.x, c1);
-                c2 = mfma16d(wa.y, qv.y, c2);
-                c3 = mfma16d(wa.x + wa.y, qv.x + qv.y, c3);
-            }
-            __syncthreads();  // every wave has read Ts (as W) before it is overwritten
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                Ts[16 * tm + 4 * r + lq][16 * tn + l16] =
-                    make_double2(c1[r] - c2[r], c3[r] - c1[r] - c2[r]);
-            __syncthreads();
-        }
-        const double2(*Out)[33] = fb ? Ts : Ws;  // fa only: the result is W
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int r0 = 16 * (t >> 1), c0 = 16 * (t & 1);
-            if (lower[t]) {
-                const double2 v = Out[r0 + jj][c0 + ii];
-                Hg[gaddr[t]] = make_double2(v.x, -v.y);
-            } else {
-                Hg[gaddr[t]] = Out[r0 + ii][c0 + jj];
-            }
-        }
+        htile_update(p, ws, round, P, P2, Ts, Ws);
         return;
     }
     // Y rows of pivot P, 16 columns per wave
-    if (part == 1) return;
     const int c0 = (y - nt) * 64;
-    if (c0 >= p.Cp || *convp || !flag[P]) return;
-    const double2 *Qa = Qall + (int64_t)P * 1024;
+    const int *flag = qflag_ptr(ws, p, round);
+    if (c0 >= p.Cp || ctl_ptr(ws, p)->conv || !flag[P]) return;
+    const double2 *Qa = qmat_ptr(ws, p, round) + (int64_t)P * 1024;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
     double2 *Z = z_ptr<double>(ws, p);
     const int LD = p.LD, R = p.R;
     const int c = c0 + 16 * wave + l16;
@@ -1259,12 +1269,74 @@ k_hupdate(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, in
     hupdate_body(ops, owner, ws, round, part, (int)blockIdx.y, Ts, Ws);
 }
 
+// The eigensolve of pivot Pn of round + 1, preceded by the round `round`
+// update of the one H tile that holds its cross block: blocks In < Jn of Pn
+// were held by pivots pa and pb of this round, and the tile between them is
+// updated here (when pa != pb; when pa == pb that pivot's eigensolve has
+// written the block). For nb >= 6 no such tile holds the cross blocks of two
+// pivots of round + 1 (the circle method sends the two blocks of a pivot to
+// pivots two slots apart), so this workgroup is the only one in the launch
+// that reads or writes the tile; problems with nb = 4 use the one-sided
+// method. The diagonal block tiles (In, In) and (Jn, Jn) were written by this
+// round's eigensolves, in the launch before; the cross block is taken from
+// the updated tile in LDS. The tile is updated in the
+// orientation part 0 uses (P2 = P + d mod n, 1 <= d <= n/2), so that its
+// arithmetic does not depend on which launch updates it.
+MPSX_DEV void hnext_body(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
+                         double tol_inner, int inner_sweeps, double abs_rel, double2 (*A)[33],
+                         double2 (*B)[33], double *prm_c, double2 *prm_s, int &s_any) {
+    constexpr int B2 = 32, b = 16;
+    const int pi = owner[blockIdx.x];
+    const mpsx_plan p = ops[pi].plan;
+    if (p.method != 1 || round >= p.nb - 1) return;
+    const int nb = p.nb, n = nb / 2, nt = n / 2;
+    const int Pn = (int)(blockIdx.x - p.pair_base);
+    int In, Jn;
+    rr_pair(Pn, round + 1, nb, &In, &Jn);
+    const bool solve = round + 1 < nb - 1;
+    const int pa = rr_pivot(In, round, nb), pb = rr_pivot(Jn, round, nb);
+    const HTile *Out = nullptr;
+    int P = pa, P2 = pb;
+    if (pa != pb) {
+        const int d = (pb - pa + n) % n;
+        const bool keep = 2 * d < n || (2 * d == n && pa < nt);
+        P = keep ? pa : pb;
+        P2 = keep ? pb : pa;
+        Out = htile_update(p, ws, round, P, P2, A, B);
+    }
+    if (!solve) return;
+    // loaded after the update (registers), the cross block replaced from LDS
+    double2 hv[HSOLVE_NE];
+    hsolve_load(h_ptr(ws, p), p.Cp, In, Jn, hv);
+    if (Out) {
+        int I, J, I2, J2;
+        rr_pair(P, round, nb, &I, &J);
+        rr_pair(P2, round, nb, &I2, &J2);
+        const bool in_p = In == I || In == J;  // In a block of P (rows of Out), Jn of P2
+#pragma unroll
+        for (int t = 0; t < HSOLVE_NE; ++t) {
+            const int e = threadIdx.x + HSOLVE_THREADS * t, i = e / B2, j = e % B2;
+            if (i >= b || j < b) continue;
+            const int jj = j - b;
+            if (in_p) {
+                hv[t] = Out[(In == J ? b : 0) + i][(Jn == J2 ? b : 0) + jj];
+            } else {  // H[In][Jn] = H[Jn][In]^H
+                const double2 v = Out[(Jn == J ? b : 0) + jj][(In == J2 ? b : 0) + i];
+                hv[t] = make_double2(v.x, -v.y);
+            }
+        }
+    }
+    if (ctl_ptr(ws, p)->conv) return;
+    __syncthreads();  // the tile in LDS has been read before the eigensolve reuses A and B
+    hsolve_core(p, ws, round + 1, Pn, In, Jn, hv, tol, tol_inner, inner_sweeps, abs_rel, A, B, prm_c, prm_s,
+                s_any);
+}
+
 // Rounds r + 1 and r in one launch: y = 0 runs the eigensolves of round
-// r + 1, y >= 1 the part-2 update of round r (its tiles that no eigensolve of
-// round r + 1 reads, and the rows of Y). The two touch disjoint tiles of H
-// and different sets of Q and flags, so they overlap freely; the part-1
-// update of round r (launched before) has written every tile the
-// eigensolves read. LDS is shared between the two roles.
+// r + 1 (each after the update of the tile holding its cross block), y >= 1
+// the rest of the update of round r (the other tiles, and the rows of Y).
+// The two touch disjoint tiles of H and different sets of Q and flags, so
+// they overlap freely. LDS is shared between the two roles.
 __global__ void __launch_bounds__(256)
 k_hround(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
          double tol_inner, int inner_sweeps, double abs_rel) {
@@ -1274,8 +1346,7 @@ k_hround(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, dou
     __shared__ double2 prm_s[16];
     __shared__ int s_any;
     if (blockIdx.y == 0)
-        hsolve_body(ops, owner, ws, round + 1, tol, tol_inner, inner_sweeps, abs_rel, A, B, prm_c, prm_s,
-                    s_any);
+        hnext_body(ops, owner, ws, round, tol, tol_inner, inner_sweeps, abs_rel, A, B, prm_c, prm_s, s_any);
     else
         hupdate_body(ops, owner, ws, round, 2, (int)blockIdx.y - 1, A, B);
 }

@@ -230,13 +230,16 @@ def _rr_partner(item, r, n):
     return 0 if opp == 0 else ((opp - 1 + r) % (n - 1)) + 1
 
 
-@pytest.mark.parametrize("nb", [4, 6, 10, 32, 34, 192])
+@pytest.mark.parametrize("nb", [6, 8, 10, 32, 34, 192])
 def test_gram_pipelined_round_split(nb):
-    """The pipelined Gram round: the update tiles k_hupdate marks critical
-    (part 1: some block tile of the pivot-pair tile is a pivot pair of the
-    next round, found with rr_partner) together with the pivot tiles k_hsolve
-    writes hold every tile the next round's eigensolves read, so the rest
-    (part 2) may run beside them."""
+    """The pipelined Gram round: the update tiles marked critical (some block
+    tile of the pivot-pair tile is a pivot pair of the next round, found with
+    rr_partner; k_hround's eigensolve workgroups update them, its part 2
+    skips them) together with the pivot tiles the eigensolves write hold
+    every tile the next round's eigensolves read, and each critical tile is
+    updated by exactly one eigensolve workgroup of the next round (hnext_body:
+    the tile between the pivots that held its two blocks), so no two
+    workgroups of one launch touch the same tile."""
     n = nb // 2
     for r in range(2 * (nb - 1)):  # rounds past nb - 2 wrap to the next sweep's pairing
         piv = [_rr_pair(P, r % (nb - 1), nb) for P in range(n)]
@@ -258,7 +261,7 @@ def test_gram_pipelined_round_split(nb):
                     have |= {(min(x, z), max(x, z)) for x in (I, J) for z in (I2, J2)}
         for I, J in nxt:
             assert {(I, I), (I, J), (J, J)} <= have, (nb, r, I, J)
-        # part 1 runs one workgroup per pivot of round r + 1; each critical tile
+        # one eigensolve workgroup per pivot of round r + 1; each critical tile
         # (a pair of pivots of round r) must be claimed by exactly one of them
         def pivot(item, rr):
             pos = 0 if item == 0 else ((item - 1 - rr) % (nb - 1)) + 1
@@ -274,14 +277,32 @@ def test_gram_pipelined_round_split(nb):
                 if {_rr_partner(I, r + 1, nb), _rr_partner(J, r + 1, nb)} & {I2, J2}:
                     critical.add(frozenset((P, (P + dd) % n)))
         claimed = []
-        for Pn, (In, Jn) in enumerate(nxt):
+        for In, Jn in nxt:
             pa, pb = pivot(In, r % (nb - 1)), pivot(Jn, r % (nb - 1))
-            if pa == pb:
-                continue
-            first = n
-            for x in piv[pa]:
-                if _rr_partner(x, r + 1, nb) in piv[pb]:
-                    first = min(first, pivot(x, (r + 1) % (nb - 1)))
-            if first == Pn:
-                claimed.append(frozenset((pa, pb)))
+            assert pa != pb, (nb, r, In, Jn)  # no pair repeats in consecutive rounds
+            claimed.append(frozenset((pa, pb)))
         assert len(claimed) == len(set(claimed)) and set(claimed) == critical, (nb, r)
+
+
+def test_gram_next_round_tiles_unshared():
+    """For every even block count from 6 up, the tile between the two pivots
+    of round r that hold the blocks of a pivot of round r + 1 differs for all
+    pivots of round r + 1 (so hnext_body's workgroups own their tiles); with
+    four blocks one tile holds both, which is why plan_method gives such
+    problems the one-sided method."""
+    def piv_of(nb, r):
+        out = {}
+        for P in range(nb // 2):
+            for x in _rr_pair(P, r, nb):
+                out[x] = P
+        return out
+
+    for nb in range(4, 258, 2):
+        shared = False
+        for r in range(nb - 1):
+            cur, nxt = piv_of(nb, r), (r + 1) % (nb - 1)
+            tiles = [frozenset((cur[In], cur[Jn])) for In, Jn in
+                     (_rr_pair(P, nxt, nb) for P in range(nb // 2))]
+            assert all(len(t) == 2 for t in tiles), (nb, r)
+            shared |= len(set(tiles)) != len(tiles)
+        assert shared == (nb == 4), nb
