@@ -1262,7 +1262,7 @@ MPSX_DEV void hupdate_body(const mpsx_two_site_op *ops, const int *owner, char *
     }
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 4)
 k_hupdate(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, int part) {
     __shared__ double2 Ts[32][33];
     __shared__ double2 Ws[32][33];
@@ -1336,8 +1336,10 @@ MPSX_DEV void hnext_body(const mpsx_two_site_op *ops, const int *owner, char *ws
 // r + 1 (each after the update of the tile holding its cross block), y >= 1
 // the rest of the update of round r (the other tiles, and the rows of Y).
 // The two touch disjoint tiles of H and different sets of Q and flags, so
-// they overlap freely. LDS is shared between the two roles.
-__global__ void __launch_bounds__(256)
+// they overlap freely. LDS is shared between the two roles. Four waves per
+// SIMD (at most 128 registers a lane, accumulators included) match the four
+// workgroups per CU that the LDS allows.
+__global__ void __launch_bounds__(256, 4)
 k_hround(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
          double tol_inner, int inner_sweeps, double abs_rel) {
     __shared__ double2 A[32][33];
