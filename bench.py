@@ -235,7 +235,7 @@ def run_qutrit(args, eng, svd):
     depth = args.depth or 16
     circ = brickwork_haar(48, depth, 3, maxsvals=1024, seed=4)
     flops, _, updates = circuit_work(circ)
-    svd_f, _ = svd_flops(circ, svd.block_switch or 64)
+    svd_f, _ = svd_flops(circ, svd.block_switch or 64, gram_min_rows(svd))
     kset = round_kernels(svd)
     peak = PEAK_FP32_MFMA if svd.fp32 else PEAK_FP64_MFMA
     dc = prepare(circ, eng)
@@ -364,6 +364,11 @@ def run_served(args, eng, svd):
     emit(line)
 
 
+def gram_min_rows(svd):
+    """Rows above which the updates with 32-row pivots run the Gram method."""
+    return 64 if not svd.fp32 and svd.method == 0 else 0
+
+
 def round_kernels(svd):
     """Kernels of one Jacobi round of the updates with 32-row pivots: the
     eigensolve and update launches of the Gram method (FP64 default), else
@@ -373,11 +378,12 @@ def round_kernels(svd):
     return [("k_jacobi_round32" if svd.fp32 else "k_jacobi_round64") + "<32>"]
 
 
-def svd_flops(circuit, switch=64):
+def svd_flops(circuit, switch=64, min_rows=0):
     """Golub-Reinsch SVD flops (the SVD term of the per-update work) of the
     updates with 32-row pivots (the rows to orthogonalise, d*chi_l, exceed
-    `switch`) and of the others, summed over the circuit, and the full
-    per-update flops. Returns ({"big": flops, "small": flops}, full)."""
+    `switch`, and `min_rows`: 64 for the Gram method, which serves six or
+    more 16-row blocks) and of the others, summed over the circuit, and the
+    full per-update flops. Returns ({"big": flops, "small": flops}, full)."""
     from mpsim_amd.circuits import two_site_flops
     n, d = circuit.n, circuit.d
     bonds = [1] * (n + 1)
@@ -393,7 +399,7 @@ def svd_flops(circuit, switch=64):
             f = two_site_flops(cl, cm, cr, d, k)
             M, N = d * cl, d * cr
             p, qq = max(M, N), min(M, N)
-            svd["big" if M > switch else "small"] += 4.0 * (4.0 * p * p * qq + 8.0 * p * qq * qq + 9.0 * qq ** 3)
+            svd["big" if M > max(switch, min_rows) else "small"] += 4.0 * (4.0 * p * p * qq + 8.0 * p * qq * qq + 9.0 * qq ** 3)
             full += f
             bonds[i + 1] = k
     return svd, full
@@ -445,7 +451,7 @@ def main():
         return run_qutrit(args, eng, svd)
     circ = brickwork_haar(64, 20, 2, maxsvals=256, seed=2)
     flops, bytes_, updates = circuit_work(circ)
-    svd_f, _ = svd_flops(circ, svd.block_switch or 64)
+    svd_f, _ = svd_flops(circ, svd.block_switch or 64, gram_min_rows(svd))
     kset = round_kernels(svd)
 
     dc = prepare(circ, eng)
@@ -483,8 +489,8 @@ def main():
     ms_per_step = 1e3 * total / args.steps
 
     # The roofline unit is one Jacobi round of the updates with 32-row pivots
-    # (k_hsolve + k_hupdate for the Gram method, one k_jacobi_round launch
-    # otherwise). Large two-site calls run as four concurrent groups on four
+    # (k_hsolve / k_hround + k_hupdate for the Gram method, one
+    # k_jacobi_round launch otherwise). Large two-site calls run as four concurrent groups on four
     # streams, so rounds overlap: throughput is the SVD flops over the busy
     # time of the round kernels (the union of all their launch intervals);
     # the per-round figure (flops per round / average round duration) is
