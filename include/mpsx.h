@@ -152,7 +152,8 @@ typedef struct {
                            device-side status when check_every = 0); the
                            caller zeroes it                                 */
     int32_t method;     /* MPSX_SVD_AUTO (0): the Gram method for FP64
-                           problems with b = 16, rows otherwise;
+                           problems with b = 16 and nb >= 6 (more than 64
+                           rows), rows otherwise;
                            MPSX_SVD_ROWS (1): the one-sided method for all */
     int32_t reserved;
 } mpsx_svd_opts;
