@@ -65,7 +65,7 @@ def test_gram_and_row_methods_agree():
     """The Gram method (two-sided Jacobi on H = X X^H, the default for FP64
     problems with 32-row pivots) and the one-sided row iteration
     (SvdConfig(method=1)) on the same batch, mixed sizes (16-row pivots, 32-row
-    pivots with 2 .. 32 blocks, absorb left and right): the truncated two-site
+    pivots with 2 .. 32 blocks, odd pivot counts, absorb left and right): the truncated two-site
     products and the kept singular values agree to the precision of their
     complex64 / float32 outputs (1e-6 of ||theta||, of sigma_max), the FP64
     discarded weights to 1e-10 of ||theta||^2, and both match LAPACK on theta
@@ -75,7 +75,10 @@ def test_gram_and_row_methods_agree():
     from mpsim_amd.engine import Engine, SvdConfig, TwoSiteItem
     eng = Engine.get()
     rng = np.random.RandomState(23)
-    shapes = [(4, 4, 4, 8, False), (16, 12, 16, 20, True), (64, 64, 64, 64, False),
+    # rows 2 chi: 8 (8-row blocks), 32 and 48 (nb 2 and 4: one-sided in both runs), 80
+    # (nb 6, the smallest Gram problem, 3 pivots), 128, 152 (nb 10, 5 pivots), 256, 512
+    shapes = [(4, 4, 4, 8, False), (16, 12, 16, 20, True), (24, 16, 24, 40, False),
+              (40, 32, 40, 64, True), (64, 64, 64, 64, False), (76, 64, 76, 120, False),
               (128, 96, 128, 200, True), (256, 256, 256, 256, False), (256, 128, 64, 100, False)]
     items, thetas = [], []
     for cl, cm, cr, k, left in shapes:
