@@ -93,10 +93,12 @@ def reset(dc: DeviceCircuit) -> None:
 
 def ws_bound(C: int, R: int) -> int:
     """Upper bound of the Jacobi workspace bytes of one C x R problem:
-    complex FP64 working rows [X | Y] (padded rows Cp <= C + 31), plus the
-    Gram method's Cp x max(Cp, R) Gram and 32x32 pivot rotations."""
+    complex FP64 working rows [X | Y] (padded rows Cp <= C + 31, row stride
+    <= R + Cp + 3, norms and order 12 Cp), plus the Gram method's
+    Cp x max(Cp, R) Gram and two sets of 32x32 pivot rotations (16 Cp x 64
+    bytes) and flags."""
     cp = C + 31
-    return 16 * cp * (R + cp + 4) + 16 * cp * (max(cp, R) + 33) + 1024
+    return 16 * cp * (R + cp + 4) + 16 * cp * (max(cp, R) + 64) + 2048
 
 
 def _ws_estimate(it: TwoSiteItem) -> int:

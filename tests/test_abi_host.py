@@ -81,6 +81,27 @@ def test_plan_layout_and_prefixes():
     assert nbytes > p[2]["ws_off"]
 
 
+@pytest.mark.parametrize("cl,cr,d", [(40, 40, 2), (256, 256, 2), (171, 171, 3), (171, 40, 3),
+                                     (33, 300, 2), (500, 17, 3)])
+def test_workspace_bound_covers_the_plan(cl, cr, d):
+    """batch.ws_bound (the per-problem estimate batching and admission use)
+    is at least the region the library plans for the problem (the distance
+    between two copies' ws_off), for both orientations and both SVD methods
+    (C = 513 = 16 nb - 31 is the tightest padding)."""
+    from mpsim_amd import _native as nat
+    from mpsim_amd.batch import ws_bound
+    for flags in (0, 1):
+        for method in (0, 1):
+            k = min(d * cl, d * cr)
+            tab = _table([(cl, 8, cr, d, k, flags)] * 2)
+            o = nat.SvdOpts(0, 0, 1, 0, 0.0)
+            o.method = method
+            rc, _, err = _plan(tab, o)
+            assert rc == 0, err
+            size = int(tab[1]["ws_off"] - tab[0]["ws_off"])
+            assert ws_bound(int(tab[0]["C"]), int(tab[0]["R"])) >= size, (flags, method, size)
+
+
 @pytest.mark.parametrize("row,msg", [
     ((4, 4, 4, 2, 9, 0), "k=9"),
     ((4, 4, 4, 5, 2, 0), "d=5"),
