@@ -76,10 +76,12 @@ def test_gram_and_row_methods_agree():
     eng = Engine.get()
     rng = np.random.RandomState(23)
     # rows 2 chi: 8 (8-row blocks), 32 and 48 (nb 2 and 4: one-sided in both runs), 80
-    # (nb 6, the smallest Gram problem, 3 pivots), 128, 152 (nb 10, 5 pivots), 256, 512
+    # (nb 6, the smallest Gram problem, 3 pivots), 128, 130 (padded to 160: 30 zero rows),
+    # 152 (nb 10, 5 pivots), 256, 512
     shapes = [(4, 4, 4, 8, False), (16, 12, 16, 20, True), (24, 16, 24, 40, False),
-              (40, 32, 40, 64, True), (64, 64, 64, 64, False), (76, 64, 76, 120, False),
-              (128, 96, 128, 200, True), (256, 256, 256, 256, False), (256, 128, 64, 100, False)]
+              (40, 32, 40, 64, True), (64, 64, 64, 64, False), (65, 48, 65, 100, False),
+              (76, 64, 76, 120, False), (128, 96, 128, 200, True), (256, 256, 256, 256, False),
+              (256, 128, 64, 100, False)]
     items, thetas = [], []
     for cl, cm, cr, k, left in shapes:
         A = ((rng.randn(cl, 2, cm) + 1j * rng.randn(cl, 2, cm)) / np.sqrt(4 * cm)).astype(np.complex64)
