@@ -13,7 +13,7 @@ baseline the batched form is compared with.
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+from typing import Callable, List, Optional, Sequence
 
 import numpy as np
 
@@ -40,21 +40,38 @@ class DeviceCircuit:
         return self.t >= len(self.circuit.layers)
 
 
+def check_layer(layer, n: int, where: str = "batched execution") -> None:
+    """A layer runs as one grouped launch of its one-site gates followed by
+    one of its two-site updates, so its ops must act on disjoint qudits (in
+    range) and every pair must be nearest-neighbour (i, i+1)."""
+    seen = set()
+    for q, _ in layer:
+        for s in q:
+            if not 0 <= s < n:
+                raise ValueError(f"{where}: qudit {s} out of range for {n} qudits")
+            if s in seen:
+                raise ValueError(f"{where}: qudit {s} appears in more than one op of a layer; "
+                                 "split the layer so each qudit is touched once")
+            seen.add(s)
+        if len(q) == 2 and q[1] != q[0] + 1:
+            raise ValueError(f"{where} takes nearest-neighbour pairs (i, i+1), got {tuple(q)}")
+        if len(q) not in (1, 2):
+            raise ValueError(f"{where}: ops act on one or two qudits, got {tuple(q)}")
+
+
 def _layer_plan(circuit: Circuit, gates: List[List[object]]) -> List[tuple]:
     """Per layer: the one-site ops as (site, gate address) and the two-site
-    ops as (left site, gate); checks that every pair is nearest-neighbour
-    (i, i+1). The gate matrices stay alive in DeviceCircuit.gates."""
+    ops as (left site, gate); checks each layer with check_layer. The gate
+    matrices stay alive in DeviceCircuit.gates."""
     plan = []
     for layer, glayer in zip(circuit.layers, gates):
+        check_layer(layer, circuit.n)
         one, two = [], []
         for (q, _), g in zip(layer, glayer):
             if len(q) == 1:
                 one.append((q[0], int(g.data_ptr())))
             else:
-                i, j = q
-                if j != i + 1:
-                    raise ValueError("batched execution takes nearest-neighbour pairs (i, i+1)")
-                two.append((i, g))
+                two.append((q[0], g))
         plan.append((one, two))
     return plan
 
@@ -110,10 +127,16 @@ def _ws_estimate(it: TwoSiteItem) -> int:
 
 
 def tick(engine: Engine, dcs: Sequence[DeviceCircuit], svd: Optional[SvdConfig] = None,
-         keep_discarded: bool = False, ws_budget: int = 8 << 30) -> int:
+         keep_discarded: bool = False, ws_budget: int = 8 << 30, want_s: bool = False,
+         observer: Optional[Callable] = None) -> int:
     """Advances every unfinished circuit by one layer; returns the number of
     two-site updates performed. The two-site updates of the tick go to the
-    library in as few calls as the workspace budget allows."""
+    library in as few calls as the workspace budget allows.
+
+    want_s: every update also returns its kept singular values.
+    observer(dc, i, item, result): called for every two-site update on sites
+    (i, i+1) of circuit dc after its library call, while the input tensors
+    (item.a, item.b) are still alive and before the MPS takes the outputs."""
     one, two, owners = [], [], []
     for dc in dcs:
         if dc.t >= len(dc.plan):
@@ -133,7 +156,7 @@ def tick(engine: Engine, dcs: Sequence[DeviceCircuit], svd: Optional[SvdConfig] 
             a, b = tensors[i], tensors[i + 1]
             M, N = d * a.shape[0], d * b.shape[2]
             k = min(M, N) if cap is None else min(M, N, cap)
-            two.append(TwoSiteItem(a, b, g, k))
+            two.append(TwoSiteItem(a, b, g, k, want_s=want_s))
             owners.append((dc, i))
         dc.t += 1
     for s in range(0, len(one), MAX_GROUP):
@@ -145,7 +168,9 @@ def tick(engine: Engine, dcs: Sequence[DeviceCircuit], svd: Optional[SvdConfig] 
             acc += _ws_estimate(two[end])
             end += 1
         res = engine.two_site(two[start:end], svd)
-        for (dc, i), r in zip(owners[start:end], res):
+        for (dc, i), it, r in zip(owners[start:end], two[start:end], res):
+            if observer is not None:
+                observer(dc, i, it, r)
             dc.mps._tensors[i] = r.a_out
             dc.mps._tensors[i + 1] = r.b_out
             if keep_discarded:

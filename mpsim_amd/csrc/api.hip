@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "common.h"
+#include "driver.h"
 #include "launch.h"
 
 namespace {
@@ -112,11 +113,19 @@ Profiler &prof() {
     return p;
 }
 
+// Set while a driver call runs that must create no events (may_poll false).
+thread_local bool g_no_profile = false;
+struct NoProfile {
+    bool prev;
+    explicit NoProfile(bool on) : prev(g_no_profile) { g_no_profile = g_no_profile || on; }
+    ~NoProfile() { g_no_profile = prev; }
+};
+
 // Launches `fn` and, when profiling is on, brackets it with events.
 template <class F>
 hipError_t timed(const char *name, hipStream_t s, F &&fn) {
     Profiler &p = prof();
-    if (!p.on) return fn();
+    if (!p.on || g_no_profile) return fn();
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
         return fn();  // events inside a captured graph would not be replay-timed
@@ -269,62 +278,85 @@ int mpsx_gemm(mpsx_stream stream, const mpsx_gemm_op *h, const mpsx_gemm_op *d, 
 }
 
 // ---------------------------------------------------------------- two-site
-int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts,
-                       size_t *bytes) {
-    if (count < 0) return fail(MPSX_EINVAL, "count < 0");
-    if (count > 0 && !h) return fail(MPSX_EINVAL, "null descriptor table");
-    const Resolved r = resolve(opts);
+}  // extern "C"
+
+namespace {
+
+// Validates op i and fills the shape fields of its plan (everything but the
+// five prefix offsets) from (d, chi_l, chi_m, chi_r, k, flags) and the
+// resolved options. The planner and the check in mpsx_two_site share it, so
+// a table planned for other shapes, flags or options is caught before
+// anything is launched.
+int plan_shape(const mpsx_two_site_op &o, int i, const Resolved &r, mpsx_plan *p) {
+    const bool split = o.gate == nullptr;
+    if (split ? o.d != 1 : (o.d < 2 || o.d > 4))
+        return fail(MPSX_EINVAL, "op %d: d=%d not in {2,3,4} (or 1 for a plain split)", i, o.d);
+    if (o.chi_l < 0 || o.chi_m < 0 || o.chi_r < 0)
+        return fail(MPSX_EINVAL, "op %d: negative bond dimension", i);
+    const int64_t M = (int64_t)o.d * o.chi_l, N = (int64_t)o.d * o.chi_r;
+    if (o.k < 0 || o.k > std::min(M, N))
+        return fail(MPSX_EINVAL, "op %d: k=%d outside [0, min(%lld,%lld)]", i, o.k, (long long)M,
+                    (long long)N);
+    if (M > 65536 || N > 65536) return fail(MPSX_EINVAL, "op %d: matrix too large", i);
+    if (M * N > 0 && !split && o.chi_m > 0 && (!o.a || !o.b))
+        return fail(MPSX_EINVAL, "op %d: null input", i);
+    if (M * N > 0 && split && !o.a) return fail(MPSX_EINVAL, "op %d: null input", i);
+    if (M * o.k > 0 && !o.a_out) return fail(MPSX_EINVAL, "op %d: null a_out", i);
+    if (N * o.k > 0 && !o.b_out) return fail(MPSX_EINVAL, "op %d: null b_out", i);
+    if (o.flags & ~1) return fail(MPSX_EINVAL, "op %d: unknown flag bits 0x%x", i, o.flags);
+    const bool left_absorb = (o.flags & 1) != 0;
+    std::memset(p, 0, sizeof *p);
+    p->C = (int32_t)(left_absorb ? N : M);
+    p->R = (int32_t)(left_absorb ? M : N);
+    p->b = r.block ? r.block : auto_block(p->C, r.block_switch);
+    if (p->C > 0) {
+        int nb = (p->C + p->b - 1) / p->b;
+        if (nb < 2) nb = 2;
+        if (nb & 1) ++nb;
+        p->nb = nb;
+        p->Cp = nb * p->b;
+    }
+    p->LD = (int32_t)align_up((int64_t)p->R + p->Cp, 4);
+    p->zbytes = r.zbytes;
+    p->method = p->nb > 0 ? plan_method(r, p->b, p->nb) : 0;
+    return MPSX_OK;
+}
+
+int check_options(const Resolved &r) {
     if (r.block != 0 && r.block != 8 && r.block != 16)
         return fail(MPSX_EINVAL, "block %d not in {0 (auto), 8, 16}", r.block);
     if (r.max_sweeps < 1 || r.max_sweeps > 64) return fail(MPSX_EINVAL, "max_sweeps outside [1,64]");
     if (r.method != MPSX_SVD_AUTO && r.method != MPSX_SVD_ROWS)
         return fail(MPSX_EINVAL, "method %d not in {0 (auto), 1 (rows)}", r.method);
+    return MPSX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts,
+                       size_t *bytes) {
+    if (count < 0) return fail(MPSX_EINVAL, "count < 0");
+    if (count > 0 && !h) return fail(MPSX_EINVAL, "null descriptor table");
+    const Resolved r = resolve(opts);
+    if (int e = check_options(r)) return e;
     int64_t off = kHeaderBytes, tiles = 0, pairs = 0, rows = 0, outs = 0;
     for (int i = 0; i < count; ++i) {
         mpsx_two_site_op &o = h[i];
-        const bool split = o.gate == nullptr;
-        if (split ? o.d != 1 : (o.d < 2 || o.d > 4))
-            return fail(MPSX_EINVAL, "op %d: d=%d not in {2,3,4} (or 1 for a plain split)", i, o.d);
-        if (o.chi_l < 0 || o.chi_m < 0 || o.chi_r < 0)
-            return fail(MPSX_EINVAL, "op %d: negative bond dimension", i);
-        const int64_t M = (int64_t)o.d * o.chi_l, N = (int64_t)o.d * o.chi_r;
-        if (o.k < 0 || o.k > std::min(M, N))
-            return fail(MPSX_EINVAL, "op %d: k=%d outside [0, min(%lld,%lld)]", i, o.k,
-                        (long long)M, (long long)N);
-        if (M > 65536 || N > 65536) return fail(MPSX_EINVAL, "op %d: matrix too large", i);
-        if (M * N > 0 && !split && o.chi_m > 0 && (!o.a || !o.b))
-            return fail(MPSX_EINVAL, "op %d: null input", i);
-        if (M * N > 0 && split && !o.a) return fail(MPSX_EINVAL, "op %d: null input", i);
-        if (M * o.k > 0 && !o.a_out) return fail(MPSX_EINVAL, "op %d: null a_out", i);
-        if (N * o.k > 0 && !o.b_out) return fail(MPSX_EINVAL, "op %d: null b_out", i);
-        const bool left_absorb = (o.flags & 1) != 0;
-        mpsx_plan &p = o.plan;
-        p.C = (int32_t)(left_absorb ? N : M);
-        p.R = (int32_t)(left_absorb ? M : N);
-        p.b = r.block ? r.block : auto_block(p.C, r.block_switch);
-        if (p.C == 0) {
-            p.nb = 0;
-            p.Cp = 0;
-        } else {
-            int nb = (p.C + p.b - 1) / p.b;
-            if (nb < 2) nb = 2;
-            if (nb & 1) ++nb;
-            p.nb = nb;
-            p.Cp = nb * p.b;
-        }
-        p.LD = (int32_t)align_up((int64_t)p.R + p.Cp, 4);
-        p.zbytes = r.zbytes;
-        p.method = p.nb > 0 ? plan_method(r, p.b, p.nb) : 0;
+        mpsx_plan p;
+        if (int e = plan_shape(o, i, r, &p)) return e;
         p.ws_off = off;
         p.tile_base = tiles;
         p.pair_base = pairs;
         p.row_base = rows;
         p.out_base = outs;
+        o.plan = p;
         off += problem_bytes(p);
         tiles += theta_tiles(o);
         pairs += p.nb / 2;
         rows += p.Cp;
-        outs += gather_units(M, o.k);
+        outs += gather_units((int64_t)o.d * o.chi_l, o.k);
     }
     if (tiles > kGridLimit || pairs > kGridLimit || rows > kGridLimit || outs > kGridLimit)
         return fail(MPSX_EINVAL, "batch too large for one launch");
@@ -335,20 +367,26 @@ int mpsx_two_site_plan(mpsx_two_site_op *h, int count, const mpsx_svd_opts *opts
     return MPSX_OK;
 }
 
-int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two_site_op *d,
-                  int count, const mpsx_svd_opts *opts, void *ws_, size_t ws_bytes,
-                  int *sweeps_out) {
-    hipStream_t stream = (hipStream_t)stream_;
+}  // extern "C"
+
+namespace {
+
+// The two-site driver behind mpsx_two_site. `may_poll` false: never read
+// device memory back (no convergence poll; the sweep limit and the
+// device-side early exit bound the work) and create no profiling events.
+int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_site_op *d,
+                 int count, const Resolved &r, char *ws, size_t ws_bytes, int *sweeps_out,
+                 bool may_poll) {
     if (sweeps_out) *sweeps_out = 0;
     if (count < 0) return fail(MPSX_EINVAL, "count < 0");
     if (count == 0) return MPSX_OK;
     if (!h || !d) return fail(MPSX_EINVAL, "null descriptor table");
-    if (!ws_) return fail(MPSX_EINVAL, "null workspace");
-    const Resolved r = resolve(opts);
-    char *ws = static_cast<char *>(ws_);
-
-    // Re-derive totals from the planned host table and check the plan is
-    // consistent (a table that was not planned, or edited, is rejected).
+    if (!ws) return fail(MPSX_EINVAL, "null workspace");
+    if (int e = check_options(r)) return e;
+    NoProfile quiet(!may_poll);
+    // Re-plan every op from its shape, flags and the options and compare with
+    // the table's plan: a table that was not planned, was planned for other
+    // options, or was edited after planning is rejected.
     int64_t need = kHeaderBytes, tiles = 0, pairs = 0, rows = 0, outs = 0;
     int rows_nb = 0, gram_nb = 0, gram_y = 0, gram_c = 0;
     bool has_d[5] = {false, false, false, false, false};  // index 1: plain splits
@@ -357,15 +395,17 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
     for (int i = 0; i < count; ++i) {
         const mpsx_two_site_op &o = h[i];
         const mpsx_plan &p = o.plan;
-        if (p.ws_off != need || p.tile_base != tiles || p.pair_base != pairs || p.row_base != rows ||
-            p.out_base != outs)
-            return fail(MPSX_EINVAL, "op %d: plan missing or stale (call mpsx_two_site_plan)", i);
-        if (o.d < 1 || o.d > 4 || (p.b != 8 && p.b != 16))
-            return fail(MPSX_EINVAL, "op %d: invalid plan", i);
-        if (p.zbytes != r.zbytes)
-            return fail(MPSX_EINVAL, "op %d: planned for the other rotation precision", i);
-        if (p.method != (p.nb > 0 ? plan_method(r, p.b, p.nb) : 0))
-            return fail(MPSX_EINVAL, "op %d: planned for another SVD method", i);
+        mpsx_plan want;
+        if (int e = plan_shape(o, i, r, &want)) return e;
+        want.ws_off = need;
+        want.tile_base = tiles;
+        want.pair_base = pairs;
+        want.row_base = rows;
+        want.out_base = outs;
+        if (std::memcmp(&want, &p, sizeof want) != 0)
+            return fail(MPSX_EINVAL,
+                        "op %d: plan missing or stale (call mpsx_two_site_plan with the same "
+                        "shapes, flags and options)", i);
         need += problem_bytes(p);
         tiles += theta_tiles(o);
         pairs += p.nb / 2;
@@ -383,6 +423,8 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
             gram_c = std::max(gram_c, (p.R + 31) / 32);
         }
     }
+    if (tiles > kGridLimit || pairs > kGridLimit || rows > kGridLimit || outs > kGridLimit)
+        return fail(MPSX_EINVAL, "batch too large for one launch");
     const int64_t owner_off = align_up(need, 256);
     need = owner_off + align_up(4 * pairs, 256);
     if ((int64_t)ws_bytes < need)
@@ -391,7 +433,7 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
 
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     HIP_TRY(hipStreamIsCapturing(stream, &cap), "hipStreamIsCapturing");
-    const bool can_poll = r.check_every > 0 && cap == hipStreamCaptureStatusNone;
+    const bool can_poll = may_poll && r.check_every > 0 && cap == hipStreamCaptureStatusNone;
 
     const bool f32 = r.fp32;
     if (rows > 0) HIP_TRY(timed("k_init_rows", stream, [&] { return launch_init_rows(stream, f32, d, count, ws, rows); }), "k_init_rows");
@@ -480,6 +522,28 @@ int mpsx_two_site(mpsx_stream stream_, const mpsx_two_site_op *h, const mpsx_two
         return fail(MPSX_ENOTCONV, "Jacobi SVD not converged after %d sweeps", sweeps);
     return MPSX_OK;
 }
+
+}  // namespace
+
+// Library-internal entry of the driver for the other source files
+// (driver.h); hidden from the shared library's exports.
+int mpsx_internal_two_site(hipStream_t stream, const mpsx_two_site_op *h,
+                           const mpsx_two_site_op *d, int count, const mpsx_svd_opts *opts,
+                           char *ws, size_t ws_bytes, int *sweeps_out, bool may_poll) {
+    return two_site_run(stream, h, d, count, resolve(opts), ws, ws_bytes, sweeps_out, may_poll);
+}
+
+int mpsx_internal_fail(int code, const char *msg) { return fail(code, "%s", msg); }
+
+extern "C" {
+
+int mpsx_two_site(mpsx_stream stream, const mpsx_two_site_op *h, const mpsx_two_site_op *d,
+                  int count, const mpsx_svd_opts *opts, void *ws, size_t ws_bytes,
+                  int *sweeps_out) {
+    return two_site_run((hipStream_t)stream, h, d, count, resolve(opts), static_cast<char *>(ws),
+                        ws_bytes, sweeps_out, true);
+}
+
 
 // ---------------------------------------------------------------- graphs
 struct mpsx_graph_s {

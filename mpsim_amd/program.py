@@ -37,7 +37,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _native as nat
-from .batch import _ws_estimate, upload_gates
+from .batch import _ws_estimate, check_layer, upload_gates
 from .circuits import Circuit, bond_plan
 from .core import MPS
 from .engine import MAX_OPS, Engine, SvdConfig, TwoSiteItem
@@ -122,15 +122,14 @@ class CircuitProgram:
                 before = [1] + (self._plans[ci][t - 1] if t else [1] * (c.n - 1)) + [1]
                 after = [1] + self._plans[ci][t] + [1]
                 offs = self._site_off[ci]
+                check_layer(c.layers[t], c.n, "a program")
                 for (q, _), g in zip(c.layers[t], self.gates[ci][t]):
                     if len(q) == 1:
                         s = q[0]
                         p = self._addr(offs[s])
                         one.append((p, p, nat.ptr(g), before[s], before[s + 1], d, 0))
                     else:
-                        i, j = q
-                        if j != i + 1:
-                            raise ValueError("a program takes nearest-neighbour pairs (i, i+1)")
+                        i = q[0]
                         cl, cm, cr, k = before[i], before[i + 1], before[i + 2], after[i + 1]
                         two.append(dict(a=self._addr(offs[i]), b=self._addr(offs[i + 1]),
                                         gate=nat.ptr(g), cl=cl, cm=cm, cr=cr, d=d, k=k,
@@ -189,7 +188,7 @@ class CircuitProgram:
         return SvdConfig(block=c.block, max_sweeps=sweeps if sweeps else c.max_sweeps,
                          check_every=0 if sweeps else 1, inner_sweeps=c.inner_sweeps, tol=c.tol,
                          block_switch=c.block_switch, unconverged=int(self.unconverged.data_ptr()),
-                         fp32=c.fp32).to_c()
+                         fp32=c.fp32, method=c.method).to_c()
 
     def _plan(self, tab: np.ndarray) -> int:
         nbytes = ctypes.c_size_t(0)

@@ -171,6 +171,7 @@ class OracleMPS:
         self._max_bond = max_bond_dimensions(nqudits, d)
         self.norms: List[float] = []  # core.py:243, appended at core.py:1161
         self.discarded: List[float] = []
+        self.spectra: List[np.ndarray] = []  # kept singular values of every update (FP64)
 
     # -- dtype handling ---------------------------------------------------
     def _cast_gate(self, g: np.ndarray) -> np.ndarray:
@@ -278,6 +279,7 @@ class OracleMPS:
         u, s, vh, rest = tn_svd_split(m, maxsvals)
         k = s.size
         self.discarded.append(float(np.sum(np.abs(rest) ** 2)))
+        self.spectra.append(np.abs(s).astype(np.float64))
         # core.py:1139-1145: absorb S right (default) or left
         if keep_left:
             left = u

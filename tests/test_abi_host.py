@@ -130,6 +130,59 @@ def test_two_site_rejects_unplanned_table():
     assert rc == 1 and "plan" in lib.mpsx_last_error().decode()
 
 
+def _run_unlaunchable(tab, opts=None):
+    """mpsx_two_site with a 1 KiB workspace: every check runs, and a table
+    that passes them stops at the workspace check (MPSX_EWORKSPACE) before
+    any HIP call."""
+    from mpsim_amd import _native as nat
+    lib = nat.load()
+    o = opts or nat.SvdOpts(0, 0, 1, 0, 0.0)
+    sweeps = ctypes.c_int(0)
+    rc = lib.mpsx_two_site(None, tab.ctypes.data, tab.ctypes.data, len(tab), ctypes.byref(o),
+                           ctypes.c_void_p(0x1000), 1024, ctypes.byref(sweeps))
+    return rc, lib.mpsx_last_error().decode()
+
+
+@pytest.mark.parametrize("field,value", [("flags", 1), ("chi_l", 3), ("chi_r", 5), ("d", 3),
+                                         ("k", 100), ("chi_m", 1)])
+def test_two_site_rejects_table_edited_after_planning(field, value):
+    """The library re-plans every op from (d, chi_l, chi_r, k, flags) and the
+    options and rejects a plan that does not match, also for the last op of
+    the table (whose edits change no later prefix). k is re-checked against
+    min(d chi_l, d chi_r); chi_m is not part of the plan (only the theta
+    kernel reads it) and is accepted."""
+    from mpsim_amd import _native as nat
+    tab = _table([(4, 4, 2, 2, 4, 0), (8, 4, 3, 2, 6, 0)])
+    rc, _, err = _plan(tab)
+    assert rc == 0, err
+    rc, err = _run_unlaunchable(tab.copy())
+    assert rc == nat.MPSX_EWORKSPACE, err
+    for i in (0, 1):
+        bad = tab.copy()
+        bad[i][field] = value
+        rc, err = _run_unlaunchable(bad)
+        if field == "chi_m":
+            assert rc == nat.MPSX_EWORKSPACE, err
+        else:
+            assert rc == nat.MPSX_EINVAL and ("stale" in err or "k=" in err), (i, field, err)
+
+
+def test_two_site_rejects_plan_for_other_options():
+    """A table planned with one SVD method or precision is rejected when the
+    call asks for another."""
+    from mpsim_amd import _native as nat
+    tab = _table([(64, 64, 64, 2, 64, 0)])
+    rc, _, err = _plan(tab)
+    assert rc == 0 and int(tab[0]["method"]) == 1, err
+    rows = nat.SvdOpts(0, 0, 1, 0, 0.0)
+    rows.method = 1
+    rc, err = _run_unlaunchable(tab, rows)
+    assert rc == nat.MPSX_EINVAL and "stale" in err, err
+    f32 = nat.SvdOpts(0, 0, 1, 0, 0.0, 0, 1)
+    rc, err = _run_unlaunchable(tab, f32)
+    assert rc == nat.MPSX_EINVAL and "stale" in err, err
+
+
 def test_haar_generator_is_the_reference_recipe_bit_for_bit():
     from oracle import mps_oracle as mo
     from mpsim_amd import gates
@@ -327,3 +380,47 @@ def test_gram_next_round_tiles_unshared():
             assert all(len(t) == 2 for t in tiles), (nb, r)
             shared |= len(set(tiles)) != len(tiles)
         assert shared == (nb == 4), nb
+
+
+def test_layers_must_touch_each_qudit_once():
+    """A layer runs as one grouped one-site launch, then one grouped two-site
+    launch: overlapping ops in a layer (two pairs sharing a qudit, a one-site
+    gate on a qudit a pair also touches), non-adjacent pairs and qudits out
+    of range are rejected before anything runs."""
+    import torch
+    from mpsim_amd.batch import _layer_plan, check_layer
+    from mpsim_amd.circuits import Circuit
+    u2, u4 = np.eye(2), np.eye(4)
+    check_layer([((0, 1), u4), ((2, 3), u4), ((4,), u2)], 5)
+    bad = [
+        [((0, 1), u4), ((1, 2), u4)],          # pairs share qudit 1
+        [((0, 1), u4), ((1,), u2)],            # one-site gate on a pair's qudit
+        [((2,), u2), ((2,), u2)],              # the same qudit twice
+        [((0, 2), u4)],                        # not nearest-neighbour
+        [((3, 4), u4)],                        # out of range for 4 qudits
+    ]
+    for layer in bad:
+        with pytest.raises(ValueError):
+            check_layer(layer, 4)
+        circ = Circuit(4, 2, [layer])
+        gates = [[torch.zeros(m.shape, dtype=torch.complex64) for _, m in layer]]
+        with pytest.raises(ValueError):
+            _layer_plan(circ, gates)
+
+
+def test_program_passes_the_svd_method():
+    """CircuitProgram plans and launches with the SvdConfig it was given,
+    method included (a one-sided request must not silently run the Gram
+    method)."""
+    import types
+    import torch
+    from mpsim_amd.engine import SvdConfig
+    from mpsim_amd.program import CircuitProgram
+    stub = types.SimpleNamespace(svd=SvdConfig(method=1, fp32=False, block_switch=16),
+                                 unconverged=torch.zeros(1, dtype=torch.int32))
+    for sweeps in (0, 7):
+        o = CircuitProgram._opts(stub, sweeps)
+        assert o.method == 1 and o.block_switch == 16
+    tab = _table([(64, 64, 64, 2, 64, 0)])
+    rc, _, err = _plan(tab, CircuitProgram._opts(stub, 0))
+    assert rc == 0 and int(tab[0]["method"]) == 0, err
