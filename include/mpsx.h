@@ -18,7 +18,8 @@
  *   - `stream` is a hipStream_t (NULL = the default stream). Calls enqueue
  *     work and return; they never synchronise the stream unless stated.
  *   - The library allocates no device memory. Scratch space is passed in by
- *     the caller (`ws`, `ws_bytes`), sized by the matching *_workspace call.
+ *     the caller (`ws`, `ws_bytes`), sized by the matching *_plan call; the
+ *     one exception is mpsx_two_site_static's fixed per-device array.
  *   - Return value: 0 on success, otherwise an MPSX_E* code; the message is
  *     available from mpsx_last_error() on the calling thread.
  */
@@ -173,6 +174,33 @@ int mpsx_two_site(mpsx_stream stream, const mpsx_two_site_op *h,
                   const mpsx_two_site_op *d, int count,
                   const mpsx_svd_opts *opts, void *ws, size_t ws_bytes,
                   int *sweeps_out);
+
+/* ------------------------------------------------------------------------
+ * mpsx_two_site on a fixed, library-owned device workspace.
+ * No counterpart in the reference (which allocates through numpy on every
+ * gate). For callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc. It
+ * takes the same planned tables as mpsx_two_site but no workspace: the
+ * library keeps one MPSX_STATIC_WS_BYTES (64 MiB) array per device for the
+ * life of the process, looked up on the first call on a device (the
+ * stream's device must be current). A batch whose planned workspace
+ * (mpsx_two_site_plan's *bytes) exceeds it is rejected with MPSX_EWORKSPACE
+ * before anything is enqueued; 64 MiB holds, e.g., five updates at
+ * chi = 256 (theta 512 x 512) or one at chi = 512 (d = 2).
+ * The call allocates nothing (no hipMalloc, hipHostMalloc, event or
+ * library allocation) and never reads device memory back: it does not poll
+ * convergence, it launches opts->max_sweeps sweeps (default 30) in which
+ * converged problems exit at once, so the results are bitwise those of
+ * mpsx_two_site; non-convergence is reported only through
+ * opts->unconverged. All calls on one device share its array: calls that
+ * overlap on one device (different streams or threads) race and corrupt
+ * each other's results; serialise them.
+ * ---------------------------------------------------------------------- */
+#define MPSX_STATIC_WS_BYTES ((size_t)64 << 20)
+
+int mpsx_two_site_static(mpsx_stream stream, const mpsx_two_site_op *h,
+                         const mpsx_two_site_op *d, int count,
+                         const mpsx_svd_opts *opts, int *sweeps_out);
 
 /* ------------------------------------------------------------------------
  * Grouped complex GEMM used by the read-out contractions:

@@ -19,6 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("MPSX_LIB", os.path.join(HERE, "libmpsx.so"))
 
 MPSX_OK, MPSX_EINVAL, MPSX_EHIP, MPSX_EWORKSPACE, MPSX_ENOTCONV = 0, 1, 2, 3, 4
+MPSX_STATIC_WS_BYTES = 64 << 20  # fixed workspace of mpsx_two_site_static
 
 SITE_OP = np.dtype([("a", "<u8"), ("out", "<u8"), ("gate", "<u8"), ("chi_l", "<i8"),
                     ("chi_r", "<i8"), ("d", "<i4"), ("reserved", "<i4")])
@@ -62,7 +63,8 @@ class NotConvergedWarning(RuntimeWarning):
 
 
 EXPORTS = ("mpsx_version", "mpsx_last_error", "mpsx_device_info", "mpsx_sizeof",
-           "mpsx_apply_one_site", "mpsx_two_site_plan", "mpsx_two_site", "mpsx_gemm",
+           "mpsx_apply_one_site", "mpsx_two_site_plan", "mpsx_two_site", "mpsx_two_site_static",
+           "mpsx_gemm",
            "mpsx_scale", "mpsx_profile_enable", "mpsx_profile_read", "mpsx_profile_busy",
            "mpsx_graph_begin", "mpsx_graph_end", "mpsx_graph_launch", "mpsx_graph_destroy")
 
@@ -90,6 +92,8 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
         lib.mpsx_two_site_plan.argtypes = [vp, i32, ctypes.POINTER(SvdOpts), ctypes.POINTER(sz)]
         lib.mpsx_two_site.argtypes = [vp, vp, vp, i32, ctypes.POINTER(SvdOpts), vp, sz,
                                       ctypes.POINTER(i32)]
+        lib.mpsx_two_site_static.argtypes = [vp, vp, vp, i32, ctypes.POINTER(SvdOpts),
+                                             ctypes.POINTER(i32)]
         lib.mpsx_gemm.argtypes = [vp, vp, vp, i32]
         lib.mpsx_scale.argtypes = [vp, vp, vp, i32]
         lib.mpsx_profile_enable.argtypes = [i32]
@@ -101,7 +105,8 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
         lib.mpsx_graph_end.argtypes = [vp, ctypes.POINTER(vp)]
         lib.mpsx_graph_launch.argtypes = [vp, vp]
         lib.mpsx_graph_destroy.argtypes = [vp]
-        for name in ("mpsx_apply_one_site", "mpsx_two_site_plan", "mpsx_two_site", "mpsx_gemm",
+        for name in ("mpsx_apply_one_site", "mpsx_two_site_plan", "mpsx_two_site",
+                     "mpsx_two_site_static", "mpsx_gemm",
                      "mpsx_scale", "mpsx_device_info", "mpsx_profile_enable",
                      "mpsx_profile_read", "mpsx_profile_busy", "mpsx_graph_begin", "mpsx_graph_end",
                      "mpsx_graph_launch", "mpsx_graph_destroy"):

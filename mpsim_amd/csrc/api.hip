@@ -376,7 +376,7 @@ namespace {
 // device-side early exit bound the work) and create no profiling events.
 int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_site_op *d,
                  int count, const Resolved &r, char *ws, size_t ws_bytes, int *sweeps_out,
-                 bool may_poll) {
+                 bool may_poll, size_t *need_out = nullptr) {
     if (sweeps_out) *sweeps_out = 0;
     if (count < 0) return fail(MPSX_EINVAL, "count < 0");
     if (count == 0) return MPSX_OK;
@@ -427,6 +427,7 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
         return fail(MPSX_EINVAL, "batch too large for one launch");
     const int64_t owner_off = align_up(need, 256);
     need = owner_off + align_up(4 * pairs, 256);
+    if (need_out) *need_out = (size_t)need;
     if ((int64_t)ws_bytes < need)
         return fail(MPSX_EWORKSPACE, "workspace %zu bytes < %lld needed", ws_bytes, (long long)need);
     int *owner = reinterpret_cast<int *>(ws + owner_off);
@@ -529,8 +530,10 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
 // (driver.h); hidden from the shared library's exports.
 int mpsx_internal_two_site(hipStream_t stream, const mpsx_two_site_op *h,
                            const mpsx_two_site_op *d, int count, const mpsx_svd_opts *opts,
-                           char *ws, size_t ws_bytes, int *sweeps_out, bool may_poll) {
-    return two_site_run(stream, h, d, count, resolve(opts), ws, ws_bytes, sweeps_out, may_poll);
+                           char *ws, size_t ws_bytes, int *sweeps_out, bool may_poll,
+                           size_t *need_out) {
+    return two_site_run(stream, h, d, count, resolve(opts), ws, ws_bytes, sweeps_out, may_poll,
+                        need_out);
 }
 
 int mpsx_internal_fail(int code, const char *msg) { return fail(code, "%s", msg); }

@@ -196,9 +196,11 @@ class Engine:
         del dtab
 
     # ------------------------------------------------------------ two-site
-    def _prepare_two_site(self, items: Sequence[TwoSiteItem], cfg, slot: int = 0):
-        """Outputs, planned host table, device table and workspace of one
-        library call (allocations on the calling thread's current stream)."""
+    def _prepare_two_site(self, items: Sequence[TwoSiteItem], cfg, slot: int = 0,
+                          workspace: bool = True):
+        """Outputs, planned host table, device table and workspace (None with
+        workspace=False) of one library call (allocations on the calling
+        thread's current stream)."""
         torch = self.torch
         n = len(items)
         # shapes and input pointers (one pass), then every output carved out of
@@ -251,7 +253,7 @@ class Engine:
         nbytes = ctypes.c_size_t(0)
         nat.check(self.lib.mpsx_two_site_plan(tab.ctypes.data, n, ctypes.byref(cfg),
                                               ctypes.byref(nbytes)))
-        ws = self.workspace(nbytes.value, slot)
+        ws = self.workspace(nbytes.value, slot) if workspace else None
         dtab = nat.to_device_table(tab, self.dev)
         return outs, tab, dtab, ws
 
