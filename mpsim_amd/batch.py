@@ -180,11 +180,12 @@ def tick(engine: Engine, dcs: Sequence[DeviceCircuit], svd: Optional[SvdConfig] 
 
 
 def run_batched(dcs: Sequence[DeviceCircuit], engine: Optional[Engine] = None,
-                svd: Optional[SvdConfig] = None) -> int:
+                svd: Optional[SvdConfig] = None, **tick_kw) -> int:
+    """Ticks until every circuit is done (tick_kw go to every tick)."""
     eng = engine or Engine.get()
     total = 0
     while any(not dc.done for dc in dcs):
-        total += tick(eng, dcs, svd)
+        total += tick(eng, dcs, svd, **tick_kw)
     return total
 
 

@@ -285,26 +285,74 @@ def test_plain_config_16_qubits_vs_statevector():
     assert max(mps.bond_dimensions()) == 256
 
 
+def _recorded_updates(m, circ, cap, left):
+    """Runs circ's nearest-neighbour two-site ops through the public API
+    (MPS.apply_two_qudit_gate) and, on a second MPS, through the same engine
+    call with want_s, recording every update's kept singular values and
+    discarded weight. Both runs must end bitwise equal."""
+    import torch
+    from mpsim_amd.engine import TwoSiteItem
+    d = circ.d
+    api = m.MPS(circ.n, d)
+    rec_mps = m.MPS(circ.n, d)
+    eng = rec_mps._engine
+    rec = []
+    for qudits, mat in circ.ops:
+        i, j = qudits
+        assert j == i + 1
+        g = np.asarray(mat).reshape([d] * 4)
+        api.apply_two_qudit_gate(g, i, j, maxsvals=cap, keep_left_canonical=left)
+        a, b = rec_mps._tensors[i], rec_mps._tensors[j]
+        k = min(d * a.shape[0], d * b.shape[2], cap)
+        r = eng.two_site([TwoSiteItem(a, b, eng.gate(g.reshape(d * d, d * d)), k,
+                                      absorb_left=not left, want_s=True)])[0]
+        rec_mps._tensors[i], rec_mps._tensors[j] = r.a_out, r.b_out
+        rec.append((r.s.cpu().numpy().astype(np.float64), float(r.discarded.cpu())))
+    torch.cuda.synchronize()
+    for x, y in zip(api.tensors, rec_mps.tensors):
+        assert np.array_equal(x.cpu().numpy(), y.cpu().numpy())
+    return api, rec
+
+
+def _check_spectra(rec, ora):
+    """Every update's kept singular values within 1e-4 relative plus 1e-6 of
+    ||theta||_F (theta is formed from complex64 tensors in FP32; the FP64
+    oracle forms it exactly), and its discarded weight within 1e-6
+    ||theta||_F^2, against the FP64 oracle's split of the same update."""
+    assert len(rec) == len(ora.spectra)
+    worst = 0.0
+    for (s, disc), s_ref, d_ref in zip(rec, ora.spectra, ora.discarded):
+        assert s.shape == s_ref.shape
+        fro = np.sqrt(np.sum(s_ref ** 2) + d_ref)
+        err = np.abs(s - s_ref)
+        assert np.all(err <= 1e-4 * s_ref + 1e-6 * fro)
+        assert abs(disc - d_ref) <= 1e-6 * fro ** 2
+        if s.size:
+            worst = max(worst, float(np.max(err / np.maximum(s_ref, 1e-30 + 1e-3 * s_ref[0]))))
+    return worst
+
+
 @pytest.mark.parametrize("left", [True, False])
 def test_truncated_spectra_vs_fp64_oracle(left):
-    """Truncated brickwork (scaled-down single-large-circuit config): the
-    state matches the FP64 restatement gauge-invariantly (overlap) and the
-    discarded weights agree."""
+    """Truncated brickwork (scaled-down single-large-circuit workload, 12
+    qubits, maxsvals 8): the state matches the FP64 restatement
+    gauge-invariantly (overlap error <= 1e-4, norms), and every update's kept
+    singular values and discarded weight agree with the oracle's split
+    (_check_spectra)."""
     m = _mpsim()
     from mpsim_amd.circuits import brickwork_haar
     from oracle.mps_oracle import gauge_invariant_error
     circ = brickwork_haar(12, 8, 2, maxsvals=8, seed=21)
     ora = OracleMPS(12, 2, dtype_mode="c128")
-    mps = m.MPS(12)
     for qudits, mat in circ.ops:
         ora.apply_two_qudit_gate(mat.reshape(2, 2, 2, 2), *qudits, maxsvals=8,
                                  keep_left_canonical=left)
-        mps.apply_two_qudit_gate(mat.reshape(2, 2, 2, 2), *qudits, maxsvals=8,
-                                 keep_left_canonical=left)
+    mps, rec = _recorded_updates(m, circ, 8, left)
     assert mps.bond_dimensions() == ora.bond_dimensions()
     err, nr, nx = gauge_invariant_error(ora.tensors, [t.cpu().numpy() for t in mps.tensors])
     assert err <= 1e-4, err
     assert abs(nr - nx) <= 1e-4 * nr
+    print(f"left={left}: overlap error {err:.2e}, worst kept-value error {_check_spectra(rec, ora):.2e}")
 
 
 def test_qutrit_untruncated_vs_statevector():
@@ -324,21 +372,24 @@ def test_qutrit_untruncated_vs_statevector():
 @pytest.mark.parametrize("left", [True, False])
 def test_qutrit_truncated_vs_fp64_oracle(left):
     """Scaled-down precision stress: 10 qutrits, depth 8, maxsvals 20;
-    gauge-invariant overlap error <= 1e-4 and kept spectra of the last
-    update within 1e-4 relative of the FP64 restatement."""
+    gauge-invariant overlap error <= 1e-4, and every update's kept singular
+    values and discarded weight against the FP64 oracle's split
+    (_check_spectra)."""
     m = _mpsim()
     from mpsim_amd.circuits import brickwork_haar
     from oracle.mps_oracle import gauge_invariant_error
     circ = brickwork_haar(10, 8, 3, maxsvals=20, seed=42)
     ora = OracleMPS(10, 3, dtype_mode="c128")
-    mps = m.MPS(10, 3)
     for qudits, mat in circ.ops:
-        g = mat.reshape(3, 3, 3, 3)
-        ora.apply_two_qudit_gate(g, *qudits, maxsvals=20, keep_left_canonical=left)
-        mps.apply_two_qudit_gate(g, *qudits, maxsvals=20, keep_left_canonical=left)
+        ora.apply_two_qudit_gate(mat.reshape(3, 3, 3, 3), *qudits, maxsvals=20,
+                                 keep_left_canonical=left)
+    mps, rec = _recorded_updates(m, circ, 20, left)
     assert mps.bond_dimensions() == ora.bond_dimensions()
     err, nr, nx = gauge_invariant_error(ora.tensors, [t.cpu().numpy() for t in mps.tensors])
     assert err <= 1e-4, err
+    assert abs(nr - nx) <= 1e-4 * nr
+    print(f"qutrits left={left}: overlap error {err:.2e}, "
+          f"worst kept-value error {_check_spectra(rec, ora):.2e}")
 
 
 def test_singular_values_of_one_update_vs_lapack():
