@@ -51,7 +51,14 @@ def _capped_mps(n: int, d: int, cap: int, mode: str, track_norms: bool,
 
 
 def time_updates(updates: int = 8, n: int = 64, d: int = 2, cap: int = 256,
-                 mode: str = "reference", track_norms: bool = True, seed: int = 2) -> Dict:
+                 mode: str = "reference", track_norms: bool = True, seed: int = 2,
+                 threads: int = 0) -> Dict:
+    """`updates` two-site updates at the cap shape; threads > 0 limits the
+    BLAS/LAPACK threads for the timed part (threadpoolctl)."""
+    if threads:
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(limits=threads, user_api="blas"):
+            return time_updates(updates, n, d, cap, mode, track_norms, seed)
     rng = np.random.RandomState(seed)
     mps = _capped_mps(n, d, cap, mode, track_norms, rng)
     mid = n // 2
@@ -74,3 +81,22 @@ def cpu_model() -> str:
     except OSError:
         pass
     return platform.processor() or "unknown"
+
+
+def time_plain16(n: int = 16, depth: int = 8, seed: int = 1) -> Dict:
+    """The 16-qubit plain workload in full, reference-faithful: every layer a
+    Haar U(2) (gates.py:269-285 recipe) on each qubit, then CNOT on (i, i+1)
+    at offset (layer mod 2), no truncation, tensors complex64 until the
+    float64 CNOT promotes them, per-gate full norm (core.py:1160-1161)."""
+    rng = np.random.RandomState(seed)
+    layers = []
+    cnot = np.array([[1.0, 0, 0, 0], [0, 1.0, 0, 0], [0, 0, 0, 1.0], [0, 0, 1.0, 0]])
+    for t in range(depth):
+        layers.append([((q,), haar_unitary_from_rng(2, rng)) for q in range(n)])
+        layers.append([((i, i + 1), cnot) for i in range(t % 2, n - 1, 2)])
+    mps = OracleMPS(n, 2, dtype_mode="reference", track_norms=True)
+    t0 = time.perf_counter()
+    for layer in layers:
+        mps.apply_ops(layer)
+    dt = time.perf_counter() - t0
+    return {"seconds": dt, "updates": sum(len(l) for l in layers[1::2])}
