@@ -48,6 +48,8 @@ def parse():
                     help="concurrent stream groups of a large two-site call (0 = engine default)")
     ap.add_argument("--fp32", action="store_true",
                     help="FP32 Jacobi working matrix (faster, ~1e-6 relative error per update)")
+    ap.add_argument("--fp32-sweeps", type=int, default=0,
+                    help="Gram method: > 0 adds an FP32 phase of at most this many sweeps")
     ap.add_argument("--workload", default="large",
                     choices=["large", "uneven", "served", "plain16", "qutrit", "copy", "accuracy",
                              "hbm"],
@@ -253,7 +255,7 @@ def run_qutrit(args, eng, svd):
     nat.profile_enable(False)
     busy = nat.profile_busy(kset)
     kern = nat.profile_read(reset=True)
-    jr_n = sum(kern.get(k, (0.0, 0))[1] for k in kset[:2]) if len(kset) == 3 else kern.get(kset[0], (0.0, 0))[1]
+    jr_n = round_launches(kern, kset)
     ach = svd_f["big"] / (busy * 1e-3) / 1e12 if busy else 0.0
     line = base_line("2-site gate updates/s, 48 qutrits chi<=1024", updates / dt, "2-site updates/s",
                      args, 1e3 * dt, {"workload": f"48 qutrits, brickwork depth {depth}, 9x9 Haar, "
@@ -438,11 +440,20 @@ def gram_min_rows(svd):
 
 def round_kernels(svd):
     """Kernels of one Jacobi round of the updates with 32-row pivots: the
-    eigensolve and update launches of the Gram method (FP64 default), else
-    the one-sided round kernel."""
+    eigensolve and update launches of the Gram method (the default; with
+    fp32_sweeps > 0 also those of its FP32 phase), else the one-sided round
+    kernel."""
     if not svd.fp32 and svd.method == 0:
-        return ["k_hsolve", "k_hround", "k_hupdate"]
+        ks = ["k_hsolve", "k_hround", "k_hupdate"]
+        return ks + [k + "32" for k in ks] if svd.fp32_sweeps > 0 else ks
     return [("k_jacobi_round32" if svd.fp32 else "k_jacobi_round64") + "<32>"]
+
+
+def round_launches(kern, kset):
+    """Rounds run: one eigensolve launch (k_hsolve* or k_hround*) per round
+    of the Gram method, one launch per round of the one-sided method."""
+    eig = [k for k in kset if k.startswith("k_hsolve") or k.startswith("k_hround")] or kset
+    return sum(kern.get(k, (0.0, 0))[1] for k in eig)
 
 
 def svd_flops(circuit, switch=64, min_rows=0):
@@ -502,7 +513,8 @@ def main():
     import dataclasses
     svd = dataclasses.replace(eng.svd, block=args.block or eng.svd.block,
                               inner_sweeps=args.inner or eng.svd.inner_sweeps, fp32=args.fp32,
-                              method=args.svd_method, tol=args.tol or eng.svd.tol)
+                              method=args.svd_method, tol=args.tol or eng.svd.tol,
+                              fp32_sweeps=args.fp32_sweeps)
     peak = PEAK_FP32_MFMA if args.fp32 else PEAK_FP64_MFMA
     if args.workload == "copy":
         return run_copy(args, eng)
@@ -566,7 +578,7 @@ def main():
     jr_ms = sum(kern.get(k, (0.0, 0))[0] for k in kset)
     # rounds: one eigensolve launch each (k_hsolve opens a sweep, k_hround runs
     # the next round's eigensolves beside the current round's update)
-    jr_n = sum(kern.get(k, (0.0, 0))[1] for k in kset[:2]) if len(kset) == 3 else kern.get(kset[0], (0.0, 0))[1]
+    jr_n = round_launches(kern, kset)
     dom_f = svd_f["big"] * args.steps
     achieved = dom_f / (jr_busy * 1e-3) / 1e12 if jr_busy > 0 else 0.0
     per_round = dom_f / (jr_ms * 1e-3) / 1e12 if jr_ms > 0 else 0.0

@@ -156,7 +156,15 @@ typedef struct {
                            problems with b = 16 and nb >= 6 (more than 64
                            rows), rows otherwise;
                            MPSX_SVD_ROWS (1): the one-sided method for all */
-    int32_t reserved;
+    int32_t fp32_sweeps;/* > 0: the Gram method starts with an FP32 phase of
+                           at most this many sweeps: its rounds keep H in
+                           FP32 on the FP32 matrix cores (rotations, Y and
+                           the convergence tests in FP64) until a sweep
+                           rotates nothing, then H is formed again in FP64
+                           from the rotated rows and the FP64 rounds finish
+                           (max_sweeps bounds those). 0 (default) or < 0:
+                           FP64 rounds only. The results agree with the FP64
+                           rounds' to FP64 rounding of the converged rows. */
 } mpsx_svd_opts;
 
 #define MPSX_SVD_AUTO 0

@@ -112,10 +112,11 @@ def ws_bound(C: int, R: int) -> int:
     """Upper bound of the Jacobi workspace bytes of one C x R problem:
     complex FP64 working rows [X | Y] (padded rows Cp <= C + 31, row stride
     <= R + Cp + 3, norms and order 12 Cp), plus the Gram method's
-    Cp x max(Cp, R) Gram and two sets of 32x32 pivot rotations (16 Cp x 64
-    bytes) and flags."""
+    Cp x max(Cp, R) Gram, two sets of 32x32 pivot rotations (16 Cp x 64
+    bytes) and flags, and the mixed Gram method's second Cp x max(Cp, R)
+    region (the FP32 Gram, then X1 = Y X)."""
     cp = C + 31
-    return 16 * cp * (R + cp + 4) + 16 * cp * (max(cp, R) + 64) + 2048
+    return 16 * cp * (R + cp + 4) + 32 * cp * max(cp, R) + 16 * cp * 64 + 2560
 
 
 def _ws_estimate(it: TwoSiteItem) -> int:

@@ -52,6 +52,10 @@ constexpr int64_t kGridLimit = 2147483647LL;
 // Absolute floor of the Gram method's rotation test, relative to
 // ||theta||_F^2: above the FP64 rounding that H gathers over the rounds.
 constexpr double kGramAbsRel = 1e-12;
+// The FP32 phase of the mixed Gram method (opt-in: opts.fp32_sweeps > 0):
+// relative tolerance and absolute floor (above the FP32 rounding H32
+// gathers; pairs below it are left to the FP64 rounds).
+constexpr double kGramTol32 = 1e-5, kGramAbsRel32 = 3e-8;
 
 struct Resolved {
     int block, max_sweeps, check_every, inner_sweeps;
@@ -62,11 +66,12 @@ struct Resolved {
     double null_rel;
     int zbytes;
     int method;
+    int fp32_sweeps;  // sweep limit of the Gram method's FP32 phase; 0: no such phase
 };
 
 Resolved resolve(const mpsx_svd_opts *o) {
     Resolved r{0, kDefaultMaxSweeps, 1, kDefaultInnerSweeps, 0.0, 64, nullptr, false, 0.0, 16,
-               MPSX_SVD_AUTO};
+               MPSX_SVD_AUTO, 0};
     if (o) {
         if (o->block) r.block = o->block;
         if (o->max_sweeps) r.max_sweeps = o->max_sweeps;
@@ -77,6 +82,7 @@ Resolved resolve(const mpsx_svd_opts *o) {
         r.unconverged = o->unconverged;
         r.fp32 = o->rotation_fp32 != 0;
         r.method = o->method;
+        r.fp32_sweeps = o->fp32_sweeps > 0 ? o->fp32_sweeps : 0;
     }
     if (r.tol <= 0) r.tol = r.fp32 ? kDefaultTol32 : kDefaultTol64;
     r.null_rel = r.fp32 ? kNullRel32 : kNullRel64;
@@ -162,9 +168,11 @@ int auto_block(int C, int switch_rows) { return C <= switch_rows ? 8 : 16; }
 // The Gram method serves FP64 problems with 32-row pivots and at least six
 // 16-row blocks unless the one-sided method is asked for (with four blocks
 // one H tile holds the cross blocks of both pivots of the next round, which
-// the pipelined rounds do not allow).
+// the pipelined rounds do not allow): method 1, or method 2 with an FP32
+// phase before the FP64 rounds when opts.fp32_sweeps asks for one.
 int plan_method(const Resolved &r, int b, int nb) {
-    return (!r.fp32 && b == 16 && nb >= 6 && r.method != MPSX_SVD_ROWS) ? 1 : 0;
+    if (r.fp32 || b != 16 || nb < 6 || r.method == MPSX_SVD_ROWS) return 0;
+    return r.fp32_sweeps > 0 ? 2 : 1;
 }
 
 int *poll_buffer() {
@@ -328,6 +336,7 @@ int check_options(const Resolved &r) {
     if (r.max_sweeps < 1 || r.max_sweeps > 64) return fail(MPSX_EINVAL, "max_sweeps outside [1,64]");
     if (r.method != MPSX_SVD_AUTO && r.method != MPSX_SVD_ROWS)
         return fail(MPSX_EINVAL, "method %d not in {0 (auto), 1 (rows)}", r.method);
+    if (r.fp32_sweeps > 64) return fail(MPSX_EINVAL, "fp32_sweeps above 64");
     return MPSX_OK;
 }
 
@@ -391,7 +400,7 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
     int rows_nb = 0, gram_nb = 0, gram_y = 0, gram_c = 0;
     bool has_d[5] = {false, false, false, false, false};  // index 1: plain splits
     bool has_b[17] = {false};  // one-sided problems by block width
-    bool has_gram = false;
+    bool has_gram = false, has_mixed = false;
     for (int i = 0; i < count; ++i) {
         const mpsx_two_site_op &o = h[i];
         const mpsx_plan &p = o.plan;
@@ -416,8 +425,9 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
             has_b[p.b] = true;
             rows_nb = std::max(rows_nb, p.nb);
         }
-        if (p.nb > 0 && p.method == 1) {
+        if (p.nb > 0 && p.method >= 1) {
             has_gram = true;
+            has_mixed = has_mixed || p.method == 2;
             gram_nb = std::max(gram_nb, p.nb);
             gram_y = std::max(gram_y, p.nb / 4 + (p.Cp + 63) / 64);
             gram_c = std::max(gram_c, (p.R + 31) / 32);
@@ -450,9 +460,53 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
     if (pairs > 0)
         HIP_TRY(timed("k_owner", stream, [&] { return launch_owner(stream, d, count, owner); }), "k_owner");
     if (has_gram)
-        HIP_TRY(timed("k_hgram", stream, [&] { return launch_hgram(stream, d, owner, ws, pairs, gram_nb); }), "k_hgram");
+        HIP_TRY(timed("k_hgram", stream, [&] { return launch_hgram(stream, d, owner, ws, pairs, gram_nb, 0); }), "k_hgram");
+    // sweeps of both phases so far: indexes the header's active counters
+    int sweep_idx = 0;
+    int *pin = (pairs > 0 && can_poll) ? poll_buffer() : nullptr;
+    auto poll_done = [&](int idx, bool *done) -> int {
+        const TwoSiteHeader *hdr = reinterpret_cast<const TwoSiteHeader *>(ws);
+        HIP_TRY(hipMemcpyAsync(pin, &hdr->active[idx & 63], sizeof(int), hipMemcpyDeviceToHost, stream),
+                "hipMemcpyAsync");
+        HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        *done = *pin == 0;
+        return MPSX_OK;
+    };
+    if (has_mixed) {
+        // FP32 phase of the mixed Gram problems: the same pipelined rounds on
+        // H32 (FP32 tiles on the FP32 matrix cores; FP64 eigensolves, Q and
+        // Y), until a sweep rotates nothing or the phase's sweep limit; then
+        // X1 = Y X and H = X1 X1^H in FP64 for the FP64 rounds below.
+        const double ti = kGramTol32 * 0.01;
+        for (int s = 0; s < r.fp32_sweeps; ++s) {
+            HIP_TRY(timed("k_hsolve32", stream,
+                          [&] {
+                              return launch_hsolve(stream, true, d, owner, ws, pairs, 0, kGramTol32, ti,
+                                                   r.inner_sweeps, kGramAbsRel32);
+                          }),
+                    "k_hsolve32");
+            for (int round = 0; round < gram_nb - 2; ++round)
+                HIP_TRY(timed("k_hround32", stream,
+                              [&] {
+                                  return launch_hround(stream, true, d, owner, ws, pairs, gram_y, round,
+                                                       kGramTol32, ti, r.inner_sweeps, kGramAbsRel32);
+                              }),
+                        "k_hround32");
+            HIP_TRY(timed("k_hupdate32", stream,
+                          [&] { return launch_hupdate(stream, true, d, owner, ws, pairs, gram_y, gram_nb - 2, 0); }),
+                    "k_hupdate32");
+            HIP_TRY(timed("k_sweep_end", stream, [&] { return launch_sweep_end(stream, d, count, ws, sweep_idx, true); }), "k_sweep_end");
+            ++sweep_idx;
+            if (pin && ((s + 1) % r.check_every == 0 || s + 1 == r.fp32_sweeps)) {
+                bool done = false;
+                if (int e = poll_done(sweep_idx - 1, &done)) return e;
+                if (done) break;
+            }
+        }
+        HIP_TRY(timed("k_hxgemm", stream, [&] { return launch_hxgemm(stream, d, owner, ws, pairs, gram_c, 1); }), "k_hxgemm");
+        HIP_TRY(timed("k_hgram", stream, [&] { return launch_hgram(stream, d, owner, ws, pairs, gram_nb, 1); }), "k_hgram");
+    }
     if (pairs > 0) {
-        int *pin = can_poll ? poll_buffer() : nullptr;
         for (int s = 0; s < r.max_sweeps; ++s) {
             for (int round = 0; round < rows_nb - 1; ++round)
                 for (int b : {8, 16})
@@ -473,33 +527,31 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
                 const double ti = r.tol * 0.01;
                 HIP_TRY(timed("k_hsolve", stream,
                               [&] {
-                                  return launch_hsolve(stream, d, owner, ws, pairs, 0, r.tol, ti, r.inner_sweeps,
-                                                       kGramAbsRel);
+                                  return launch_hsolve(stream, false, d, owner, ws, pairs, 0, r.tol, ti,
+                                                       r.inner_sweeps, kGramAbsRel);
                               }),
                         "k_hsolve");
                 for (int round = 0; round < gram_nb - 2; ++round) {
                     HIP_TRY(timed("k_hround", stream,
                                   [&] {
-                                      return launch_hround(stream, d, owner, ws, pairs, gram_y, round, r.tol, ti,
-                                                           r.inner_sweeps, kGramAbsRel);
+                                      return launch_hround(stream, false, d, owner, ws, pairs, gram_y, round, r.tol,
+                                                           ti, r.inner_sweeps, kGramAbsRel);
                                   }),
                             "k_hround");
                 }
                 HIP_TRY(timed("k_hupdate", stream,
                               [&] {
-                                  return launch_hupdate(stream, d, owner, ws, pairs, gram_y, gram_nb - 2, 0);
+                                  return launch_hupdate(stream, false, d, owner, ws, pairs, gram_y, gram_nb - 2, 0);
                               }),
                         "k_hupdate");
             }
-            HIP_TRY(timed("k_sweep_end", stream, [&] { return launch_sweep_end(stream, d, count, ws, s); }), "k_sweep_end");
+            HIP_TRY(timed("k_sweep_end", stream, [&] { return launch_sweep_end(stream, d, count, ws, sweep_idx, false); }), "k_sweep_end");
+            ++sweep_idx;
             ++sweeps;
             if (pin && ((s + 1) % r.check_every == 0 || s + 1 == r.max_sweeps)) {
-                const TwoSiteHeader *hdr = reinterpret_cast<const TwoSiteHeader *>(ws);
-                HIP_TRY(hipMemcpyAsync(pin, &hdr->active[s & 63], sizeof(int), hipMemcpyDeviceToHost,
-                                       stream),
-                        "hipMemcpyAsync");
-                HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
-                if (*pin == 0) {
+                bool done = false;
+                if (int e = poll_done(sweep_idx - 1, &done)) return e;
+                if (done) {
                     converged = true;
                     break;
                 }
@@ -511,7 +563,7 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
 
     if (has_gram) {
         HIP_TRY(timed("k_hxcopy", stream, [&] { return launch_hxcopy(stream, d, count, ws, rows); }), "k_hxcopy");
-        HIP_TRY(timed("k_hxgemm", stream, [&] { return launch_hxgemm(stream, d, owner, ws, pairs, gram_c); }), "k_hxgemm");
+        HIP_TRY(timed("k_hxgemm", stream, [&] { return launch_hxgemm(stream, d, owner, ws, pairs, gram_c, 0); }), "k_hxgemm");
     }
     if (rows > 0) {
         HIP_TRY(timed("k_rownorm", stream, [&] { return launch_rownorm(stream, f32, d, count, ws, rows); }), "k_rownorm");

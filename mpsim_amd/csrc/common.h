@@ -111,7 +111,7 @@ MPSX_HD int64_t ctl_bytes() { return 64; }
 MPSX_HD int64_t sig_off(const mpsx_plan &p) { return p.ws_off + ctl_bytes(); }
 MPSX_HD int64_t perm_off(const mpsx_plan &p) { return sig_off(p) + align_up(8LL * p.Cp, 256); }
 MPSX_HD int64_t z_off(const mpsx_plan &p) { return perm_off(p) + align_up(4LL * p.Cp, 256); }
-// Problems of the Gram method (plan.method 1, b = 16, FP64) add after Z:
+// Problems of the Gram method (plan.method 1 or 2, b = 16, FP64) add after Z:
 //   H[Cp][Cp] (double2, upper 16x16 block tiles valid; at the end the same
 //   region holds a copy of X as Cp x R) | Q[2][nb/2][32][32] (double2, the
 //   rotations of each pivot, for even and odd rounds) | flag[2][nb/2]
@@ -124,9 +124,18 @@ MPSX_HD int64_t h_off(const mpsx_plan &p) {
 }
 MPSX_HD int64_t qmat_off(const mpsx_plan &p) { return h_off(p) + align_up(16 * h_elems(p), 256); }
 MPSX_HD int64_t qflag_off(const mpsx_plan &p) { return qmat_off(p) + 2 * 16LL * (p.nb / 2) * 1024; }
+// Problems of the mixed Gram method (plan.method 2: an FP32 phase, then the
+// FP64 rounds) add a region T of H's size after the flags. In the FP32 phase
+// it holds H32 (Cp x Cp float2, row stride Cp, upper 16x16 block tiles
+// valid); at the switch to FP64 it receives X1 = Y X (Cp x R double2, row
+// stride R), from which H is formed again in FP64.
+MPSX_HD int64_t t_off(const mpsx_plan &p) {
+    return align_up(qflag_off(p) + 2 * 4LL * (p.nb / 2), 256);
+}
 MPSX_HD int64_t problem_bytes(const mpsx_plan &p) {
-    if (p.method != 1) return align_up(z_off(p) - p.ws_off + (int64_t)p.zbytes * p.Cp * p.LD, 256);
-    return align_up(qflag_off(p) - p.ws_off + 2 * 4LL * (p.nb / 2), 256);
+    if (p.method == 0) return align_up(z_off(p) - p.ws_off + (int64_t)p.zbytes * p.Cp * p.LD, 256);
+    if (p.method == 1) return align_up(qflag_off(p) - p.ws_off + 2 * 4LL * (p.nb / 2), 256);
+    return align_up(t_off(p) - p.ws_off + 16 * h_elems(p), 256);
 }
 
 // Complex element of the working matrix: float2 (FP32 rotations) or

@@ -25,21 +25,21 @@ hipError_t launch_jacobi_round(hipStream_t s, bool f32, int b, const mpsx_two_si
                                char *ws, int64_t pairs, int round, double tol, double tol_inner,
                                int inner_sweeps);
 hipError_t launch_sweep_end(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
-                            int sweep);
+                            int sweep, bool fp32_phase);
 hipError_t launch_sort(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
                        int *unconverged);
 hipError_t launch_gather(hipStream_t s, bool f32, const mpsx_two_site_op *d, int count, char *ws,
                          int64_t outs);
 hipError_t launch_hgram(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
-                        int64_t pairs, int max_nb);
-hipError_t launch_hsolve(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+                        int64_t pairs, int max_nb, int stage);
+hipError_t launch_hsolve(hipStream_t s, bool f32h, const mpsx_two_site_op *d, const int *owner, char *ws,
                          int64_t pairs, int round, double tol, double tol_inner, int inner_sweeps,
                          double abs_rel);
-hipError_t launch_hupdate(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+hipError_t launch_hupdate(hipStream_t s, bool f32h, const mpsx_two_site_op *d, const int *owner, char *ws,
                           int64_t pairs, int ytiles, int round, int part);
-hipError_t launch_hround(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+hipError_t launch_hround(hipStream_t s, bool f32h, const mpsx_two_site_op *d, const int *owner, char *ws,
                          int64_t pairs, int ytiles, int round, double tol, double tol_inner,
                          int inner_sweeps, double abs_rel);
 hipError_t launch_hxcopy(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int64_t rows);
 hipError_t launch_hxgemm(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
-                         int64_t pairs, int ctiles);
+                         int64_t pairs, int ctiles, int mode);

@@ -915,6 +915,38 @@ MPSX_DEV double2 *qmat_ptr(char *ws, const mpsx_plan &p, int round) {
 MPSX_DEV int *qflag_ptr(char *ws, const mpsx_plan &p, int round) {
     return reinterpret_cast<int *>(ws + qflag_off(p)) + (int64_t)(round & 1) * (p.nb / 2);
 }
+// The FP32 H of a mixed problem's first phase, and the FP64 X1 = Y X that
+// the same region receives at the switch to FP64.
+MPSX_DEV float2 *h32_ptr(char *ws, const mpsx_plan &p) {
+    return reinterpret_cast<float2 *>(ws + t_off(p));
+}
+MPSX_DEV double2 *t64_ptr(char *ws, const mpsx_plan &p) {
+    return reinterpret_cast<double2 *>(ws + t_off(p));
+}
+// Problems the Gram round kernels work on: in the FP32 phase (F32H) the
+// mixed ones (method 2), in the FP64 rounds every Gram problem (methods 1
+// and 2; a mixed problem there has passed its switch to FP64).
+template <bool F32H>
+MPSX_DEV bool gram_on(const mpsx_plan &p) {
+    return F32H ? p.method == 2 : p.method >= 1;
+}
+// Element access to H (Cp x Cp, row stride Cp) in the phase's precision;
+// values travel as double2 either way.
+template <bool F32H>
+MPSX_DEV double2 h_load(char *ws, const mpsx_plan &p, int64_t idx) {
+    if (F32H) {
+        const float2 v = h32_ptr(ws, p)[idx];
+        return make_double2(v.x, v.y);
+    }
+    return h_ptr(ws, p)[idx];
+}
+template <bool F32H>
+MPSX_DEV void h_store(char *ws, const mpsx_plan &p, int64_t idx, double2 v) {
+    if (F32H)
+        h32_ptr(ws, p)[idx] = make_float2((float)v.x, (float)v.y);
+    else
+        h_ptr(ws, p)[idx] = v;
+}
 
 // Block paired with block `item` in round r of the circle method over n
 // blocks (the inverse of rr_item, then the slot opposite).
@@ -933,12 +965,23 @@ MPSX_DEV int rr_pivot(int item, int r, int n) {
 // number j (nb + 1) + y of the nb (nb + 1) / 2 upper tiles in row-major
 // order. Waves take 16-column blocks w, w + 4, ...; three real products per
 // tile as in the round kernel's Gram; partials summed in a fixed order.
+// STAGE 0: X is the working matrix's X part (row stride LD); H goes to the
+// H region (method 1) or, rounded, to H32 (method 2). STAGE 1, the switch
+// of a mixed problem to FP64: X is X1 = Y X in region T (row stride R), H
+// goes to the H region, and the problem's convergence state is reset for
+// the FP64 sweeps.
+template <int STAGE>
 __global__ void __launch_bounds__(256)
 k_hgram(const mpsx_two_site_op *ops, const int *owner, char *ws) {
     __shared__ double2 part[4][16][17];
     const int pi = owner[blockIdx.x];
     const mpsx_plan p = ops[pi].plan;
-    if (p.method != 1) return;
+    if (STAGE == 0 ? p.method < 1 : p.method != 2) return;
+    if (STAGE == 1 && blockIdx.x == p.pair_base && blockIdx.y == 0 && threadIdx.x == 0) {
+        ProblemCtl *ctl = ctl_ptr(ws, p);
+        ctl->conv = 0;
+        ctl->counter = 0;
+    }
     const int nb = p.nb;
     if ((int)blockIdx.y > nb) return;
     int rem = (int)(blockIdx.x - p.pair_base) * (nb + 1) + (int)blockIdx.y;
@@ -950,8 +993,8 @@ k_hgram(const mpsx_two_site_op *ops, const int *owner, char *ws) {
     const int bj = bi + rem;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
-    const double2 *Z = z_ptr<double>(ws, p);
-    const int LD = p.LD, R = p.R;
+    const double2 *Z = STAGE == 0 ? z_ptr<double>(ws, p) : t64_ptr(ws, p);
+    const int LD = STAGE == 0 ? p.LD : p.R, R = p.R;
     const int64_t ra = (int64_t)(bi * 16 + l16) * LD, rb = (int64_t)(bj * 16 + l16) * LD;
     mfma_d4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = p1, p3 = p1;
     for (int c0 = wave * 16; c0 < R; c0 += 64) {
@@ -978,7 +1021,11 @@ k_hgram(const mpsx_two_site_op *ops, const int *owner, char *ws) {
     const double2 a = part[0][i][j], b = part[1][i][j], c = part[2][i][j], d = part[3][i][j];
     double2 v = make_double2((a.x + c.x) + (b.x + d.x), (a.y + c.y) + (b.y + d.y));
     if (bi == bj && i == j) v.y = 0.0;
-    h_ptr(ws, p)[(int64_t)(bi * 16 + i) * p.Cp + bj * 16 + j] = v;
+    const int64_t idx = (int64_t)(bi * 16 + i) * p.Cp + bj * 16 + j;
+    if (STAGE == 0 && p.method == 2)
+        h_store<true>(ws, p, idx, v);
+    else
+        h_ptr(ws, p)[idx] = v;
 }
 
 // Workgroup size of the eigensolve kernel (eight waves measured slower than
@@ -991,13 +1038,15 @@ static_assert((32 * 32) % HSOLVE_THREADS == 0, "pivot block loads per thread");
 // This thread's elements (i, j), i <= j, of the 32x32 pivot block of blocks
 // I < J: element e = tid + 256 t of the block in row-major order lies in a
 // stored tile, (I, I), (I, J) or (J, J), read row-contiguously.
-MPSX_DEV void hsolve_load(const double2 *Hg, int Cp, int I, int J, double2 (&hv)[HSOLVE_NE]) {
+template <bool F32H>
+MPSX_DEV void hsolve_load(char *ws, const mpsx_plan &p, int I, int J, double2 (&hv)[HSOLVE_NE]) {
     constexpr int B2 = 32, b = 16;
+    const int Cp = p.Cp;
     auto grow = [&](int i) { return i < b ? I * b + i : J * b + (i - b); };
 #pragma unroll
     for (int t = 0; t < HSOLVE_NE; ++t) {
         const int e = threadIdx.x + HSOLVE_THREADS * t, i = e / B2, j = e % B2;
-        if (i <= j) hv[t] = Hg[(int64_t)grow(i) * Cp + grow(j)];
+        if (i <= j) hv[t] = h_load<F32H>(ws, p, (int64_t)grow(i) * Cp + grow(j));
     }
 }
 
@@ -1009,6 +1058,7 @@ MPSX_DEV void hsolve_load(const double2 *Hg, int Cp, int I, int J, double2 (&hv)
 // the FP64 rounding H gathers over the rounds (pairs below it would
 // otherwise keep rotating on noise; their residual coupling changes the
 // truncated factors only to second order).
+template <bool F32H>
 MPSX_DEV void hsolve_core(const mpsx_plan &p, char *ws, int round, int pair, int I, int J,
                           const double2 (&hv)[HSOLVE_NE], double tol, double tol_inner, int inner_sweeps,
                           double abs_rel, double2 (*H)[33], double2 (*Q)[33], double *prm_c,
@@ -1016,7 +1066,6 @@ MPSX_DEV void hsolve_core(const mpsx_plan &p, char *ws, int round, int pair, int
     constexpr int B2 = 32, b = 16, NT = HSOLVE_THREADS, NE = HSOLVE_NE;
     ProblemCtl *ctl = ctl_ptr(ws, p);
     auto grow = [&](int i) { return i < b ? I * b + i : J * b + (i - b); };
-    double2 *Hg = h_ptr(ws, p);
     const int Cp = p.Cp, tid = threadIdx.x;
     // mirrored into the lower triangle
 #pragma unroll
@@ -1041,7 +1090,7 @@ MPSX_DEV void hsolve_core(const mpsx_plan &p, char *ws, int round, int pair, int
     for (int e = tid; e < B2 * B2; e += NT) {
         const int i = e / B2, j = e % B2;
         Qg[e] = Q[i][j];
-        if (i <= j) Hg[(int64_t)grow(i) * Cp + grow(j)] = H[i][j];
+        if (i <= j) h_store<F32H>(ws, p, (int64_t)grow(i) * Cp + grow(j), H[i][j]);
     }
     if (tid == 0) flag[pair] = 1;
 }
@@ -1049,23 +1098,25 @@ MPSX_DEV void hsolve_core(const mpsx_plan &p, char *ws, int round, int pair, int
 // Round `round`, eigensolves: one workgroup per pivot. Loads the pivot's
 // 32x32 block of H (all loads in flight together with the convergence flag:
 // the addresses are valid either way) and solves it.
+template <bool F32H>
 MPSX_DEV void hsolve_body(const mpsx_two_site_op *ops, const int *owner, char *ws, int round,
                           double tol, double tol_inner, int inner_sweeps, double abs_rel,
                           double2 (*H)[33], double2 (*Q)[33], double *prm_c, double2 *prm_s,
                           int &s_any) {
     const int pi = owner[blockIdx.x];
     const mpsx_plan p = ops[pi].plan;
-    if (p.method != 1 || round >= p.nb - 1) return;
+    if (!gram_on<F32H>(p) || round >= p.nb - 1) return;
     const int pair = (int)(blockIdx.x - p.pair_base);
     int I, J;
     rr_pair(pair, round, p.nb, &I, &J);
     double2 hv[HSOLVE_NE];
-    hsolve_load(h_ptr(ws, p), p.Cp, I, J, hv);
+    hsolve_load<F32H>(ws, p, I, J, hv);
     if (ctl_ptr(ws, p)->conv) return;
-    hsolve_core(p, ws, round, pair, I, J, hv, tol, tol_inner, inner_sweeps, abs_rel, H, Q, prm_c, prm_s,
-                s_any);
+    hsolve_core<F32H>(p, ws, round, pair, I, J, hv, tol, tol_inner, inner_sweeps, abs_rel, H, Q, prm_c,
+                      prm_s, s_any);
 }
 
+template <bool F32H>
 __global__ void __launch_bounds__(HSOLVE_THREADS)
 k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
          double tol_inner, int inner_sweeps, double abs_rel) {
@@ -1074,7 +1125,8 @@ k_hsolve(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, dou
     __shared__ double prm_c[16];
     __shared__ double2 prm_s[16];
     __shared__ int s_any;
-    hsolve_body(ops, owner, ws, round, tol, tol_inner, inner_sweeps, abs_rel, H, Q, prm_c, prm_s, s_any);
+    hsolve_body<F32H>(ops, owner, ws, round, tol, tol_inner, inner_sweeps, abs_rel, H, Q, prm_c, prm_s,
+                      s_any);
 }
 
 // Result of an H tile update: the LDS array holding it.
@@ -1192,6 +1244,110 @@ MPSX_DEV const HTile *htile_update(const mpsx_plan &p, char *ws, int round, int 
     return Out;
 }
 
+// The same tile update for the FP32 phase of a mixed problem: the tile is
+// read from and written to H32 and the products run on the FP32 matrix
+// cores (v_mfma_f32_16x16x4f32: operands laid out as in the FP64
+// instruction; result register r of lane l is D[4*(l/16) + r][l%16]), with
+// the FP64 rotations rounded to FP32 as they are loaded. Ts and Ws are
+// float2 views of the workgroup's LDS tiles.
+using HTile32 = float2[33];
+MPSX_DEV const HTile32 *htile_update32(const mpsx_plan &p, char *ws, int round, int P, int P2,
+                                       float2 (*Ts)[33], float2 (*Ws)[33]) {
+    constexpr int b = 16;
+    const int *convp = &ctl_ptr(ws, p)->conv;
+    const int *flag = qflag_ptr(ws, p, round);
+    const double2 *Qall = qmat_ptr(ws, p, round);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    int I, J, I2, J2;
+    rr_pair(P, round, p.nb, &I, &J);
+    rr_pair(P2, round, p.nb, &I2, &J2);
+    float2 *Hg = h32_ptr(ws, p);
+    const int Cp = p.Cp;
+    const int ii = tid >> 4, jj = tid & 15;
+    int64_t gaddr[4];
+    bool lower[4];
+    float2 tv[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int bi = (t >> 1) ? J : I, bj = (t & 1) ? J2 : I2;
+        lower[t] = bi > bj;
+        gaddr[t] = lower[t] ? (int64_t)(bj * b + ii) * Cp + bi * b + jj
+                            : (int64_t)(bi * b + ii) * Cp + bj * b + jj;
+        tv[t] = Hg[gaddr[t]];
+    }
+    const int fa = flag[P], fb = flag[P2];
+    const int tm = wave >> 1, tn = wave & 1;
+    const double2 *Qa = Qall + (int64_t)P * 1024, *Qb = Qall + (int64_t)P2 * 1024;
+    float2 qa[8], qb[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int k = 4 * s + lq;
+        const double2 a = Qa[k * 32 + 16 * tm + l16];  // conj: A = Q_P^H[16 tm + l16][k]
+        const double2 c = Qb[k * 32 + 16 * tn + l16];  // B = Q_P2[k][16 tn + l16]
+        qa[s] = make_float2((float)a.x, (float)a.y);
+        qb[s] = make_float2((float)c.x, (float)c.y);
+    }
+    if (*convp || (!fa && !fb)) return nullptr;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const float2 v = tv[t];
+        const int r0 = 16 * (t >> 1), c0 = 16 * (t & 1);
+        if (lower[t])
+            Ts[r0 + jj][c0 + ii] = make_float2(v.x, -v.y);
+        else
+            Ts[r0 + ii][c0 + jj] = v;
+    }
+    __syncthreads();
+    // W = Q_P^H T: register r of lane (l16, lq) is W[16 tm + 4 lq + r][16 tn + l16]
+    if (fa) {
+        mfma_f4 a1 = {0.f, 0.f, 0.f, 0.f}, a2 = a1, a3 = a1;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int k = 4 * s + lq;
+            const float2 q = qa[s];
+            const float2 t = Ts[k][16 * tn + l16];
+            a1 = mfma16(q.x, t.x, a1);
+            a2 = mfma16(q.y, t.y, a2);
+            a3 = mfma16(q.x - q.y, t.x + t.y, a3);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            Ws[16 * tm + 4 * lq + r][16 * tn + l16] = make_float2(a1[r] + a2[r], a3[r] - a1[r] + a2[r]);
+        __syncthreads();
+    }
+    if (fb) {
+        const float2(*Wsrc)[33] = fa ? Ws : Ts;
+        mfma_f4 c1 = {0.f, 0.f, 0.f, 0.f}, c2 = c1, c3 = c1;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int k = 4 * s + lq;
+            const float2 wa = Wsrc[16 * tm + l16][k];
+            const float2 qv = qb[s];
+            c1 = mfma16(wa.x, qv.x, c1);
+            c2 = mfma16(wa.y, qv.y, c2);
+            c3 = mfma16(wa.x + wa.y, qv.x + qv.y, c3);
+        }
+        __syncthreads();  // every wave has read Ts (as W) before it is overwritten
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            Ts[16 * tm + 4 * lq + r][16 * tn + l16] = make_float2(c1[r] - c2[r], c3[r] - c1[r] - c2[r]);
+        __syncthreads();
+    }
+    const HTile32 *Out = fb ? Ts : Ws;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int r0 = 16 * (t >> 1), c0 = 16 * (t & 1);
+        if (lower[t]) {
+            const float2 v = Out[r0 + jj][c0 + ii];
+            Hg[gaddr[t]] = make_float2(v.x, -v.y);
+        } else {
+            Hg[gaddr[t]] = Out[r0 + ii][c0 + jj];
+        }
+    }
+    return Out;
+}
+
 // Round `round`, update (`part` 0: all of it; 2: all but the tiles that hold
 // the cross block of a pivot of round + 1, which the workgroups of round
 // + 1's eigensolves update themselves, see hnext_body). x: pivot P of a
@@ -1199,12 +1355,13 @@ MPSX_DEV const HTile *htile_update(const mpsx_plan &p, char *ws, int round, int 
 // P2 = (P + y + 1) mod n, each unordered pair once; y >= n/2: columns
 // 64 (y - n/2) .. +63 of the Y rows of P, Y <- Q_P^H Y (FP64 MFMAs, wave w
 // owns 16 columns of all 32 rows).
+template <bool F32H>
 MPSX_DEV void hupdate_body(const mpsx_two_site_op *ops, const int *owner, char *ws, int round,
                            int part, int y, double2 (*Ts)[33], double2 (*Ws)[33]) {
     constexpr int b = 16;
     const int pi = owner[blockIdx.x];
     const mpsx_plan p = ops[pi].plan;
-    if (p.method != 1 || round >= p.nb - 1) return;
+    if (!gram_on<F32H>(p) || round >= p.nb - 1) return;
     const int n = p.nb / 2, nt = n / 2, nb = p.nb;
     const int P = (int)(blockIdx.x - p.pair_base);
     int I, J;
@@ -1219,10 +1376,14 @@ MPSX_DEV void hupdate_body(const mpsx_two_site_op *ops, const int *owner, char *
             const int pi_ = rr_partner(I, round + 1, nb), pj_ = rr_partner(J, round + 1, nb);
             if (pi_ == I2 || pi_ == J2 || pj_ == I2 || pj_ == J2) return;
         }
-        htile_update(p, ws, round, P, P2, Ts, Ws);
+        if (F32H)
+            htile_update32(p, ws, round, P, P2, reinterpret_cast<float2(*)[33]>(Ts),
+                           reinterpret_cast<float2(*)[33]>(Ws));
+        else
+            htile_update(p, ws, round, P, P2, Ts, Ws);
         return;
     }
-    // Y rows of pivot P, 16 columns per wave
+    // Y rows of pivot P, 16 columns per wave (FP64 in both phases)
     const int c0 = (y - nt) * 64;
     const int *flag = qflag_ptr(ws, p, round);
     if (c0 >= p.Cp || ctl_ptr(ws, p)->conv || !flag[P]) return;
@@ -1262,11 +1423,12 @@ MPSX_DEV void hupdate_body(const mpsx_two_site_op *ops, const int *owner, char *
     }
 }
 
+template <bool F32H>
 __global__ void __launch_bounds__(256, 4)
 k_hupdate(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, int part) {
     __shared__ double2 Ts[32][33];
     __shared__ double2 Ws[32][33];
-    hupdate_body(ops, owner, ws, round, part, (int)blockIdx.y, Ts, Ws);
+    hupdate_body<F32H>(ops, owner, ws, round, part, (int)blockIdx.y, Ts, Ws);
 }
 
 // The eigensolve of pivot Pn of round + 1, preceded by the round `round`
@@ -1282,13 +1444,14 @@ k_hupdate(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, in
 // the updated tile in LDS. The tile is updated in the
 // orientation part 0 uses (P2 = P + d mod n, 1 <= d <= n/2), so that its
 // arithmetic does not depend on which launch updates it.
+template <bool F32H>
 MPSX_DEV void hnext_body(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
                          double tol_inner, int inner_sweeps, double abs_rel, double2 (*A)[33],
                          double2 (*B)[33], double *prm_c, double2 *prm_s, int &s_any) {
     constexpr int B2 = 32, b = 16;
     const int pi = owner[blockIdx.x];
     const mpsx_plan p = ops[pi].plan;
-    if (p.method != 1 || round >= p.nb - 1) return;
+    if (!gram_on<F32H>(p) || round >= p.nb - 1) return;
     const int nb = p.nb, n = nb / 2, nt = n / 2;
     const int Pn = (int)(blockIdx.x - p.pair_base);
     int In, Jn;
@@ -1296,19 +1459,31 @@ MPSX_DEV void hnext_body(const mpsx_two_site_op *ops, const int *owner, char *ws
     const bool solve = round + 1 < nb - 1;
     const int pa = rr_pivot(In, round, nb), pb = rr_pivot(Jn, round, nb);
     const HTile *Out = nullptr;
+    const HTile32 *Out32 = nullptr;
     int P = pa, P2 = pb;
     if (pa != pb) {
         const int d = (pb - pa + n) % n;
         const bool keep = 2 * d < n || (2 * d == n && pa < nt);
         P = keep ? pa : pb;
         P2 = keep ? pb : pa;
-        Out = htile_update(p, ws, round, P, P2, A, B);
+        if (F32H)
+            Out32 = htile_update32(p, ws, round, P, P2, reinterpret_cast<float2(*)[33]>(A),
+                                   reinterpret_cast<float2(*)[33]>(B));
+        else
+            Out = htile_update(p, ws, round, P, P2, A, B);
     }
     if (!solve) return;
     // loaded after the update (registers), the cross block replaced from LDS
     double2 hv[HSOLVE_NE];
-    hsolve_load(h_ptr(ws, p), p.Cp, In, Jn, hv);
-    if (Out) {
+    hsolve_load<F32H>(ws, p, In, Jn, hv);
+    auto tile = [&](int r, int c) {
+        if (F32H) {
+            const float2 v = Out32[r][c];
+            return make_double2(v.x, v.y);
+        }
+        return Out[r][c];
+    };
+    if (Out || Out32) {
         int I, J, I2, J2;
         rr_pair(P, round, nb, &I, &J);
         rr_pair(P2, round, nb, &I2, &J2);
@@ -1319,17 +1494,17 @@ MPSX_DEV void hnext_body(const mpsx_two_site_op *ops, const int *owner, char *ws
             if (i >= b || j < b) continue;
             const int jj = j - b;
             if (in_p) {
-                hv[t] = Out[(In == J ? b : 0) + i][(Jn == J2 ? b : 0) + jj];
+                hv[t] = tile((In == J ? b : 0) + i, (Jn == J2 ? b : 0) + jj);
             } else {  // H[In][Jn] = H[Jn][In]^H
-                const double2 v = Out[(Jn == J ? b : 0) + jj][(In == J2 ? b : 0) + i];
+                const double2 v = tile((Jn == J ? b : 0) + jj, (In == J2 ? b : 0) + i);
                 hv[t] = make_double2(v.x, -v.y);
             }
         }
     }
     if (ctl_ptr(ws, p)->conv) return;
     __syncthreads();  // the tile in LDS has been read before the eigensolve reuses A and B
-    hsolve_core(p, ws, round + 1, Pn, In, Jn, hv, tol, tol_inner, inner_sweeps, abs_rel, A, B, prm_c, prm_s,
-                s_any);
+    hsolve_core<F32H>(p, ws, round + 1, Pn, In, Jn, hv, tol, tol_inner, inner_sweeps, abs_rel, A, B, prm_c,
+                      prm_s, s_any);
 }
 
 // Rounds r + 1 and r in one launch: y = 0 runs the eigensolves of round
@@ -1339,6 +1514,7 @@ MPSX_DEV void hnext_body(const mpsx_two_site_op *ops, const int *owner, char *ws
 // they overlap freely. LDS is shared between the two roles. Four waves per
 // SIMD (at most 128 registers a lane, accumulators included) match the four
 // workgroups per CU that the LDS allows.
+template <bool F32H>
 __global__ void __launch_bounds__(256, 4)
 k_hround(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, double tol,
          double tol_inner, int inner_sweeps, double abs_rel) {
@@ -1348,9 +1524,10 @@ k_hround(const mpsx_two_site_op *ops, const int *owner, char *ws, int round, dou
     __shared__ double2 prm_s[16];
     __shared__ int s_any;
     if (blockIdx.y == 0)
-        hnext_body(ops, owner, ws, round, tol, tol_inner, inner_sweeps, abs_rel, A, B, prm_c, prm_s, s_any);
+        hnext_body<F32H>(ops, owner, ws, round, tol, tol_inner, inner_sweeps, abs_rel, A, B, prm_c, prm_s,
+                         s_any);
     else
-        hupdate_body(ops, owner, ws, round, 2, (int)blockIdx.y - 1, A, B);
+        hupdate_body<F32H>(ops, owner, ws, round, 2, (int)blockIdx.y - 1, A, B);
 }
 
 // End of the Gram method, 1: copy X (Cp x R) into the H region (row stride
@@ -1360,7 +1537,7 @@ k_hxcopy(const mpsx_two_site_op *ops, int count, char *ws) {
     const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
                                OP_FIELD(plan.row_base), blockIdx.x);
     const mpsx_plan p = ops[pi].plan;
-    if (p.method != 1) return;
+    if (p.method < 1) return;
     const int row = (int)(blockIdx.x - p.row_base);
     if (row >= p.Cp) return;
     const double2 *z = z_ptr<double>(ws, p) + (int64_t)row * p.LD;
@@ -1372,11 +1549,14 @@ k_hxcopy(const mpsx_two_site_op *ops, int count, char *ws) {
 // are those the one-sided iteration would have produced. x: pivot slot j
 // = rows 32 j .. 32 j + 31 (Cp = 32 nb/2), y: 32 columns; wave w computes
 // the 16x16 output tile (w / 2, w % 2) over K = Cp with three real products.
+// MODE 0: that product (every Gram problem). MODE 1, the switch of a mixed
+// problem to FP64: X1 = Y X into region T (row stride R); X stays.
+template <int MODE>
 __global__ void __launch_bounds__(256)
 k_hxgemm(const mpsx_two_site_op *ops, const int *owner, char *ws) {
     const int pi = owner[blockIdx.x];
     const mpsx_plan p = ops[pi].plan;
-    if (p.method != 1) return;
+    if (MODE == 0 ? p.method < 1 : p.method != 2) return;
     const int R = p.R, LD = p.LD, Cp = p.Cp;
     const int col0 = (int)blockIdx.y * 32;
     if (col0 >= R) return;
@@ -1387,7 +1567,10 @@ k_hxgemm(const mpsx_two_site_op *ops, const int *owner, char *ws) {
     const int col = col0 + 16 * tn + l16;
     const bool ok = col < R;
     double2 *Z = z_ptr<double>(ws, p);
-    const double2 *Xc = h_ptr(ws, p);
+    const double2 *Xc = MODE == 0 ? h_ptr(ws, p) : Z;  // B = X[k][col]
+    const int ldx = MODE == 0 ? R : LD;
+    double2 *Out = MODE == 0 ? Z : t64_ptr(ws, p);
+    const int ldo = MODE == 0 ? LD : R;
     const double2 *yrow = Z + (int64_t)(row0 + l16) * LD + R;  // A = Y[row0 + l16][k]
     mfma_d4 a1 = {0.0, 0.0, 0.0, 0.0}, a2 = a1, a3 = a1;
     for (int k0 = 0; k0 < Cp; k0 += 16) {
@@ -1396,7 +1579,7 @@ k_hxgemm(const mpsx_two_site_op *ops, const int *owner, char *ws) {
         for (int s = 0; s < 4; ++s) {
             const int k = k0 + 4 * s + lq;
             ya[s] = yrow[k];
-            xb[s] = ok ? Xc[(int64_t)k * R + col] : make_double2(0.0, 0.0);
+            xb[s] = ok ? Xc[(int64_t)k * ldx + col] : make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
@@ -1408,7 +1591,7 @@ k_hxgemm(const mpsx_two_site_op *ops, const int *owner, char *ws) {
     if (ok) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-            Z[(int64_t)(row0 + 4 * r + lq) * LD + col] =
+            Out[(int64_t)(row0 + 4 * r + lq) * ldo + col] =
                 make_double2(a1[r] - a2[r], a3[r] - a1[r] - a2[r]);
     }
 }
@@ -1423,10 +1606,16 @@ __global__ void k_owner(const mpsx_two_site_op *ops, int count, int *owner) {
 }
 
 // After each sweep: a problem whose sweep rotated nothing is converged.
-__global__ void k_sweep_end(const mpsx_two_site_op *ops, int count, char *ws, int sweep) {
+// `sweep` counts the sweeps of both phases of the call (it indexes the
+// header's active counters, reused modulo 64: the next one is cleared
+// here). In the FP32 phase (fp32_phase) only the mixed Gram problems swept.
+__global__ void k_sweep_end(const mpsx_two_site_op *ops, int count, char *ws, int sweep,
+                            int fp32_phase) {
     const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi == 0) reinterpret_cast<TwoSiteHeader *>(ws)->active[(sweep + 1) & 63] = 0;
     if (pi >= count) return;
     const mpsx_plan p = ops[pi].plan;
+    if (fp32_phase && p.method != 2) return;
     ProblemCtl *ctl = ctl_ptr(ws, p);
     if (ctl->conv) return;
     if (ctl->counter == 0) {
@@ -1609,9 +1798,10 @@ hipError_t launch_jacobi_round(hipStream_t s, bool f32, int b, const mpsx_two_si
     return hipGetLastError();
 }
 
-hipError_t launch_sweep_end(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int sweep) {
+hipError_t launch_sweep_end(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int sweep,
+                            bool fp32_phase) {
     hipLaunchKernelGGL(k_sweep_end, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, d, count, ws,
-                       sweep);
+                       sweep, fp32_phase ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -1631,32 +1821,49 @@ hipError_t launch_gather(hipStream_t s, bool f32, const mpsx_two_site_op *d, int
 }
 
 hipError_t launch_hgram(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
-                        int64_t pairs, int max_nb) {
-    hipLaunchKernelGGL(k_hgram, dim3((unsigned)pairs, (unsigned)(max_nb + 1)), dim3(256), 0, s, d, owner, ws);
+                        int64_t pairs, int max_nb, int stage) {
+    const dim3 g((unsigned)pairs, (unsigned)(max_nb + 1));
+    if (stage == 0)
+        hipLaunchKernelGGL(k_hgram<0>, g, dim3(256), 0, s, d, owner, ws);
+    else
+        hipLaunchKernelGGL(k_hgram<1>, g, dim3(256), 0, s, d, owner, ws);
     return hipGetLastError();
 }
 
-hipError_t launch_hsolve(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+hipError_t launch_hsolve(hipStream_t s, bool f32h, const mpsx_two_site_op *d, const int *owner, char *ws,
                          int64_t pairs, int round, double tol, double tol_inner, int inner_sweeps,
                          double abs_rel) {
-    hipLaunchKernelGGL(k_hsolve, dim3((unsigned)pairs), dim3(HSOLVE_THREADS), 0, s, d, owner, ws, round, tol,
-                       tol_inner, inner_sweeps, abs_rel);
+    const dim3 g((unsigned)pairs), blk(HSOLVE_THREADS);
+    if (f32h)
+        hipLaunchKernelGGL(k_hsolve<true>, g, blk, 0, s, d, owner, ws, round, tol, tol_inner, inner_sweeps,
+                           abs_rel);
+    else
+        hipLaunchKernelGGL(k_hsolve<false>, g, blk, 0, s, d, owner, ws, round, tol, tol_inner, inner_sweeps,
+                           abs_rel);
     return hipGetLastError();
 }
 
-hipError_t launch_hround(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+hipError_t launch_hround(hipStream_t s, bool f32h, const mpsx_two_site_op *d, const int *owner, char *ws,
                          int64_t pairs, int ytiles, int round, double tol, double tol_inner,
                          int inner_sweeps, double abs_rel) {
     static_assert(HSOLVE_THREADS == 256, "k_hround runs both roles with one workgroup size");
-    hipLaunchKernelGGL(k_hround, dim3((unsigned)pairs, (unsigned)(ytiles + 1)), dim3(256), 0, s, d, owner, ws,
-                       round, tol, tol_inner, inner_sweeps, abs_rel);
+    const dim3 g((unsigned)pairs, (unsigned)(ytiles + 1)), blk(256);
+    if (f32h)
+        hipLaunchKernelGGL(k_hround<true>, g, blk, 0, s, d, owner, ws, round, tol, tol_inner, inner_sweeps,
+                           abs_rel);
+    else
+        hipLaunchKernelGGL(k_hround<false>, g, blk, 0, s, d, owner, ws, round, tol, tol_inner, inner_sweeps,
+                           abs_rel);
     return hipGetLastError();
 }
 
-hipError_t launch_hupdate(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
+hipError_t launch_hupdate(hipStream_t s, bool f32h, const mpsx_two_site_op *d, const int *owner, char *ws,
                           int64_t pairs, int ytiles, int round, int part) {
-    hipLaunchKernelGGL(k_hupdate, dim3((unsigned)pairs, (unsigned)ytiles), dim3(256), 0, s, d, owner,
-                       ws, round, part);
+    const dim3 g((unsigned)pairs, (unsigned)ytiles), blk(256);
+    if (f32h)
+        hipLaunchKernelGGL(k_hupdate<true>, g, blk, 0, s, d, owner, ws, round, part);
+    else
+        hipLaunchKernelGGL(k_hupdate<false>, g, blk, 0, s, d, owner, ws, round, part);
     return hipGetLastError();
 }
 
@@ -1666,7 +1873,11 @@ hipError_t launch_hxcopy(hipStream_t s, const mpsx_two_site_op *d, int count, ch
 }
 
 hipError_t launch_hxgemm(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
-                         int64_t pairs, int ctiles) {
-    hipLaunchKernelGGL(k_hxgemm, dim3((unsigned)pairs, (unsigned)ctiles), dim3(256), 0, s, d, owner, ws);
+                         int64_t pairs, int ctiles, int mode) {
+    const dim3 g((unsigned)pairs, (unsigned)ctiles);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_hxgemm<0>, g, dim3(256), 0, s, d, owner, ws);
+    else
+        hipLaunchKernelGGL(k_hxgemm<1>, g, dim3(256), 0, s, d, owner, ws);
     return hipGetLastError();
 }

@@ -67,11 +67,12 @@ class SvdConfig:
     unconverged: int = 0    # device address of an int32 non-convergence counter, or 0
     fp32: bool = False      # FP32 working matrix (faster, ~1e-6 relative error per update)
     method: int = 0         # 0: Gram method for FP64 32-row pivots; 1: one-sided everywhere
+    fp32_sweeps: int = 0    # Gram method: > 0 adds an FP32 phase of at most this many sweeps
 
     def to_c(self) -> nat.SvdOpts:
         return nat.SvdOpts(self.block, self.max_sweeps, self.check_every, self.inner_sweeps,
                            self.tol, self.block_switch, 1 if self.fp32 else 0,
-                           self.unconverged or None, self.method, 0)
+                           self.unconverged or None, self.method, self.fp32_sweeps)
 
 
 @dataclass

@@ -86,16 +86,18 @@ def test_plan_layout_and_prefixes():
 def test_workspace_bound_covers_the_plan(cl, cr, d):
     """batch.ws_bound (the per-problem estimate batching and admission use)
     is at least the region the library plans for the problem (the distance
-    between two copies' ws_off), for both orientations and both SVD methods
-    (C = 513 = 16 nb - 31 is the tightest padding)."""
+    between two copies' ws_off), for both orientations and every SVD method
+    (mixed Gram, FP64 Gram, one-sided rows; C = 513 = 16 nb - 31 is the
+    tightest padding)."""
     from mpsim_amd import _native as nat
     from mpsim_amd.batch import ws_bound
     for flags in (0, 1):
-        for method in (0, 1):
+        for method, fp32_sweeps in ((0, 0), (0, 12), (1, 0)):
             k = min(d * cl, d * cr)
             tab = _table([(cl, 8, cr, d, k, flags)] * 2)
             o = nat.SvdOpts(0, 0, 1, 0, 0.0)
             o.method = method
+            o.fp32_sweeps = fp32_sweeps
             rc, _, err = _plan(tab, o)
             assert rc == 0, err
             size = int(tab[1]["ws_off"] - tab[0]["ws_off"])
@@ -178,6 +180,12 @@ def test_two_site_rejects_plan_for_other_options():
     rows.method = 1
     rc, err = _run_unlaunchable(tab, rows)
     assert rc == nat.MPSX_EINVAL and "stale" in err, err
+    mixed = nat.SvdOpts(0, 0, 1, 0, 0.0)
+    mixed.fp32_sweeps = 12
+    rc, err = _run_unlaunchable(tab, mixed)
+    assert rc == nat.MPSX_EINVAL and "stale" in err, err
+    rc, _, err = _plan(tab, mixed)
+    assert rc == 0 and int(tab[0]["method"]) == 2, err
     f32 = nat.SvdOpts(0, 0, 1, 0, 0.0, 0, 1)
     rc, err = _run_unlaunchable(tab, f32)
     assert rc == nat.MPSX_EINVAL and "stale" in err, err

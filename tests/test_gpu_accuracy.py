@@ -110,3 +110,52 @@ def test_gram_and_row_methods_agree():
         assert np.max(np.abs(g[2] - s_ref[:k])) <= 1e-6 * s_ref[0]
         assert abs(g[3] - o[3]) <= 1e-10 * scale ** 2 + 1e-6 * abs(o[3])
         assert abs(g[3] - np.sum(s_ref[k:] ** 2)) <= 1e-6 * scale ** 2
+
+
+def test_mixed_fp32_phase_matches_fp64():
+    """The Gram method with its opt-in FP32 phase (SvdConfig(fp32_sweeps=12):
+    H in FP32 on the FP32 matrix cores until the FP32 sweeps stop rotating,
+    then H formed again in FP64 and the FP64 rounds) against the FP64 Gram
+    method on mixed sizes: truncated products within 1e-6 of ||theta||, kept
+    singular values within 1e-6 of sigma_max (and of LAPACK), discarded
+    weights within 1e-10 ||theta||^2; and a truncated circuit stays within
+    30x complex64 LAPACK of the FP64 oracle."""
+    import torch
+    from mpsim_amd import gates
+    from mpsim_amd.circuits import brickwork_haar
+    from mpsim_amd.engine import Engine, SvdConfig, TwoSiteItem
+    eng = Engine.get()
+    rng = np.random.RandomState(29)
+    shapes = [(40, 32, 40, 64, True), (64, 64, 64, 64, False), (65, 48, 65, 100, False),
+              (128, 96, 128, 200, True), (256, 256, 256, 256, False), (256, 128, 64, 100, False)]
+    items, thetas = [], []
+    for cl, cm, cr, k, left in shapes:
+        A = ((rng.randn(cl, 2, cm) + 1j * rng.randn(cl, 2, cm)) / np.sqrt(4 * cm)).astype(np.complex64)
+        B = ((rng.randn(cm, 2, cr) + 1j * rng.randn(cm, 2, cr)) / np.sqrt(2)).astype(np.complex64)
+        G = gates.haar_unitary(4, rng)
+        items.append(TwoSiteItem(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), eng.gate(G),
+                                 k, absorb_left=left, want_s=True))
+        th = np.einsum("abpq,lpqr->labr", G.reshape(2, 2, 2, 2).astype(np.complex128),
+                       np.tensordot(A.astype(np.complex128), B.astype(np.complex128), axes=([2], [0])))
+        thetas.append(th.reshape(2 * cl, 2 * cr))
+    out = {}
+    for name, cfg in (("fp64", SvdConfig(block_switch=16)),
+                      ("mixed", SvdConfig(block_switch=16, fp32_sweeps=12))):
+        res = eng.two_site(items, cfg)
+        torch.cuda.synchronize()
+        out[name] = [(r.a_out.cpu().numpy().astype(np.complex128),
+                      r.b_out.cpu().numpy().astype(np.complex128),
+                      r.s.cpu().numpy().astype(np.float64), float(r.discarded.cpu())) for r in res]
+    for (cl, cm, cr, k, left), th, f, x in zip(shapes, thetas, out["fp64"], out["mixed"]):
+        M, N = th.shape
+        scale = np.linalg.norm(th)
+        assert np.linalg.norm(f[0].reshape(M, k) @ f[1].reshape(k, N)
+                              - x[0].reshape(M, k) @ x[1].reshape(k, N)) <= 1e-6 * scale
+        s_ref = np.linalg.svd(th, compute_uv=False)
+        assert np.max(np.abs(x[2] - f[2])) <= 1e-6 * s_ref[0]
+        assert np.max(np.abs(x[2] - s_ref[:k])) <= 1e-6 * s_ref[0]
+        assert abs(x[3] - f[3]) <= 1e-10 * scale ** 2 + 1e-6 * abs(f[3])
+    circ = brickwork_haar(48, 20, 2, maxsvals=64, seed=148)
+    err, err_c64 = _overlap_errors(circ, SvdConfig(fp32_sweeps=12))
+    print(f"mixed FP32 phase: overlap error {err:.3e} (complex64 LAPACK {err_c64:.3e})")
+    assert err <= 1e-4 and err <= max(1e-9, 30 * err_c64)
