@@ -1,11 +1,14 @@
 """Layer-batched execution of many circuits on one GPU.
 
 The reference applies one gate at a time (core.py:1245, "TODO:
-Parallelize"). Here every circuit advances one layer per tick, and all the
-one-site gates of a tick (over all circuits) go to one grouped launch and
-all the two-site updates of a tick (disjoint pairs of every circuit's
-current layer) go to one grouped theta + Jacobi SVD launch sequence. Gate
-matrices are uploaded once per circuit and stay resident.
+Parallelize"). Here every circuit advances one step of its schedule per
+tick (circuits.schedule: a layer of nearest-neighbour ops on disjoint
+qudits is one step; non-adjacent ops become the reference's SWAP networks,
+overlapping ops keep their order), and all the one-site gates of a tick
+(over all circuits) go to one grouped launch and all the two-site updates of
+a tick (disjoint pairs of every circuit's current step) go to one grouped
+theta + Jacobi SVD launch sequence. Gate matrices are uploaded once per
+circuit and stay resident.
 
 `run_plain` is the one-gate-at-a-time path through the public MPS API, the
 baseline the batched form is compared with.
@@ -17,7 +20,7 @@ from typing import Callable, List, Optional, Sequence
 
 import numpy as np
 
-from .circuits import Circuit
+from .circuits import Circuit, flip_pair, schedule, swap_matrix
 from .core import MPS
 from .engine import Engine, SvdConfig, TwoSiteItem
 
@@ -29,57 +32,47 @@ class DeviceCircuit:
     """A circuit whose gates are resident on the device, with its MPS."""
     circuit: Circuit
     mps: MPS
-    gates: List[List[object]]          # per layer, per op: device matrix
-    t: int = 0                         # next layer
+    gates: List[List[object]]          # per layer, per op: device matrix (pair order applied)
+    t: int = 0                         # next step of the schedule
     discarded: List[object] = field(default_factory=list)
-    # per layer: [(site, gate)] one-site ops and [(i, gate, maxsvals)] pairs (i, i+1)
+    # per step of circuits.schedule: [(site, gate address)] one-site ops and
+    # [(i, gate)] two-site updates on (i, i+1)
     plan: List[tuple] = field(default_factory=list)
 
     @property
     def done(self) -> bool:
-        return self.t >= len(self.circuit.layers)
+        return self.t >= len(self.plan)
 
 
-def check_layer(layer, n: int, where: str = "batched execution") -> None:
-    """A layer runs as one grouped launch of its one-site gates followed by
-    one of its two-site updates, so its ops must act on disjoint qudits (in
-    range) and every pair must be nearest-neighbour (i, i+1)."""
-    seen = set()
-    for q, _ in layer:
-        for s in q:
-            if not 0 <= s < n:
-                raise ValueError(f"{where}: qudit {s} out of range for {n} qudits")
-            if s in seen:
-                raise ValueError(f"{where}: qudit {s} appears in more than one op of a layer; "
-                                 "split the layer so each qudit is touched once")
-            seen.add(s)
-        if len(q) == 2 and q[1] != q[0] + 1:
-            raise ValueError(f"{where} takes nearest-neighbour pairs (i, i+1), got {tuple(q)}")
-        if len(q) not in (1, 2):
-            raise ValueError(f"{where}: ops act on one or two qudits, got {tuple(q)}")
+def op_matrix(qudits, mat, d: int) -> np.ndarray:
+    """The complex64 matrix an op's update uses: a pair given as (j, i) with
+    j > i acts on the sites in the order (i, j) with its edges reordered
+    (core.py:1031-1033)."""
+    m = np.asarray(mat)
+    if len(qudits) == 2 and qudits[0] > qudits[1]:
+        m = flip_pair(m, d)
+    return np.ascontiguousarray(m, dtype=np.complex64)
 
 
-def _layer_plan(circuit: Circuit, gates: List[List[object]]) -> List[tuple]:
-    """Per layer: the one-site ops as (site, gate address) and the two-site
-    ops as (left site, gate); checks each layer with check_layer. The gate
-    matrices stay alive in DeviceCircuit.gates."""
+def step_plan(circuit: Circuit, gates: List[List[object]], swap_gate) -> List[tuple]:
+    """Per step of circuits.schedule: the one-site ops as (site, gate
+    address) and the two-site updates as (left site, gate); SWAP hops use
+    `swap_gate`. The gate matrices stay alive in DeviceCircuit.gates."""
+    steps, _ = schedule(circuit)
     plan = []
-    for layer, glayer in zip(circuit.layers, gates):
-        check_layer(layer, circuit.n)
-        one, two = [], []
-        for (q, _), g in zip(layer, glayer):
-            if len(q) == 1:
-                one.append((q[0], int(g.data_ptr())))
-            else:
-                two.append((q[0], g))
+    for st in steps:
+        one = [(site, int(gates[t][o].data_ptr())) for site, t, o in st.ones]
+        two = [(i, gates[t][o] if o >= 0 else swap_gate) for i, t, o, _ in st.twos]
         plan.append((one, two))
     return plan
 
 
 def upload_gates(engine: Engine, circuit: Circuit) -> List[List[object]]:
-    """One host->device copy of every gate matrix of the circuit."""
+    """One host->device copy of every gate matrix of the circuit (pairs
+    given high-to-low flipped, see op_matrix)."""
     torch = engine.torch
-    mats = [np.asarray(m, dtype=np.complex64).ravel() for layer in circuit.layers for _, m in layer]
+    d = circuit.d
+    mats = [op_matrix(q, m, d).ravel() for layer in circuit.layers for q, m in layer]
     if not mats:
         return [[] for _ in circuit.layers]
     flat = torch.from_numpy(np.concatenate(mats)).to(engine.dev)
@@ -87,7 +80,7 @@ def upload_gates(engine: Engine, circuit: Circuit) -> List[List[object]]:
     for layer in circuit.layers:
         row = []
         for q, m in layer:
-            dim = circuit.d ** len(q)
+            dim = d ** len(q)
             row.append(flat[pos:pos + dim * dim].view(dim, dim))
             pos += dim * dim
         out.append(row)
@@ -95,10 +88,14 @@ def upload_gates(engine: Engine, circuit: Circuit) -> List[List[object]]:
 
 
 def prepare(circuit: Circuit, engine: Optional[Engine] = None) -> DeviceCircuit:
+    """Uploads the circuit's gates and plans its steps (raises ValueError on
+    an op with qudits out of range, or not one or two distinct qudits)."""
     eng = engine or Engine.get()
+    schedule(circuit)  # validates before anything is allocated
     mps = MPS(circuit.n, circuit.d, device=eng.dev)
     gates = upload_gates(eng, circuit)
-    return DeviceCircuit(circuit, mps, gates, plan=_layer_plan(circuit, gates))
+    plan = step_plan(circuit, gates, eng.gate(swap_matrix(circuit.d)))
+    return DeviceCircuit(circuit, mps, gates, plan=plan)
 
 
 def reset(dc: DeviceCircuit) -> None:
@@ -130,7 +127,7 @@ def _ws_estimate(it: TwoSiteItem) -> int:
 def tick(engine: Engine, dcs: Sequence[DeviceCircuit], svd: Optional[SvdConfig] = None,
          keep_discarded: bool = False, ws_budget: int = 8 << 30, want_s: bool = False,
          observer: Optional[Callable] = None) -> int:
-    """Advances every unfinished circuit by one layer; returns the number of
+    """Advances every unfinished circuit by one step; returns the number of
     two-site updates performed. The two-site updates of the tick go to the
     library in as few calls as the workspace budget allows.
 

@@ -10,7 +10,7 @@ pairs, so a layer is one batched two-site launch.
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence, Tuple
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -20,6 +20,35 @@ Op = Tuple[Tuple[int, ...], np.ndarray]
 
 CNOT = np.array([[1.0, 0, 0, 0], [0, 1.0, 0, 0], [0, 0, 0, 1.0], [0, 0, 1.0, 0]],
                 dtype=np.complex128)
+SWAP = np.array([[1.0, 0, 0, 0], [0, 0, 1.0, 0], [0, 1.0, 0, 0], [0, 0, 0, 1.0]],
+                dtype=np.complex128)
+
+
+def swap_matrix(d: int) -> np.ndarray:
+    """SWAP of two d-level qudits as a d^2 x d^2 matrix (gates.py:204-212 for d = 2)."""
+    m = np.zeros((d * d, d * d), dtype=np.complex128)
+    for a in range(d):
+        for b in range(d):
+            m[b * d + a, a * d + b] = 1.0
+    return m
+
+
+def flip_pair(mat: np.ndarray, d: int) -> np.ndarray:
+    """The gate of a pair given as (j, i) with j > i, for the sites in the
+    order (i, j): edges reordered [1, 0, 3, 2] as core.py:1031-1033 does,
+    i.e. SWAP G SWAP."""
+    return np.asarray(mat).reshape(d, d, d, d).transpose(1, 0, 3, 2).reshape(d * d, d * d)
+
+
+@dataclass
+class Step:
+    """One grouped launch's worth of work: one-site gates and two-site
+    updates on disjoint sites. ones: (site, layer, op); twos: (left site i,
+    layer, op, flipped) for an update on (i, i+1) with that op's gate
+    (flipped: the op was given as (i+1, i)), or (i, layer, -1, False) for a
+    SWAP hop of a non-adjacent op's network."""
+    ones: List[Tuple[int, int, int]] = field(default_factory=list)
+    twos: List[Tuple[int, int, int, bool]] = field(default_factory=list)
 
 
 @dataclass
@@ -136,31 +165,92 @@ def poisson_requests(rate: float = 200.0, duration: float = 60.0, seed: int = 5,
 
 
 # ---------------------------------------------------------------- planning
-def bond_plan(circuit: Circuit) -> List[List[int]]:
-    """Bond dimensions after every layer, computed on the host from the
-    circuit alone: a two-site update on bond (i, i+1) sets it to
-    k = min(d*chi_left, d*chi_right, maxsvals) because the reference keeps
-    zero singular values (TensorNetwork split without an error cut)."""
+def schedule(circuit: Circuit) -> Tuple[List[Step], List[int]]:
+    """Expands the layers into steps of grouped launches, with the same
+    result as applying every op in order (core.py:1221-1247).
+
+    A two-qudit op on non-adjacent qudits (i, j), i < j (an op given as
+    (j, i) is flipped first), becomes the reference's SWAP network
+    (core.py:1035-1043, 1154-1219): SWAP hops (i, i+1) .. (j-2, j-1) carry
+    qudit i next to j, the gate acts on (j-1, j), and hops (j-2, j-1) ..
+    (i, i+1) carry it back; every hop is a full two-site update with the
+    circuit's truncation. Each op covers the sites from its lowest to its
+    highest qudit; an op starts once every earlier op of the layer that
+    covers one of its sites has finished, so ops on disjoint sites share
+    steps and overlapping ones keep their order. A layer of disjoint
+    nearest-neighbour ops is one step. Layers follow each other.
+
+    Returns (steps, layer_end) with layer_end[t] = number of steps after
+    layer t."""
+    steps: List[Step] = []
+    ends: List[int] = []
+    n = circuit.n
+    for t, layer in enumerate(circuit.layers):
+        base = len(steps)
+        free: Dict[int, int] = {}  # site -> first step (relative to base) it is free
+        for o, (q, _) in enumerate(layer):
+            q = tuple(int(x) for x in q)
+            if len(q) not in (1, 2) or any(not 0 <= x < n for x in q):
+                raise ValueError(f"layer {t} op {o}: qudits {q} not one or two of 0..{n - 1}")
+            if len(q) == 1:
+                lo = hi = q[0]
+                chain = [("one", q[0], o, False)]
+            else:
+                i, j = q
+                if i == j:
+                    raise ValueError(f"layer {t} op {o}: the two qudits are the same")
+                flipped = j < i
+                lo, hi = min(i, j), max(i, j)
+                there = [("two", s_, -1, False) for s_ in range(lo, hi - 1)]
+                chain = there + [("two", hi - 1, o, flipped)] + there[::-1]
+            start = max(free.get(x, 0) for x in range(lo, hi + 1))
+            while len(steps) < base + start + len(chain):
+                steps.append(Step())
+            for k, (kind, site, op, fl) in enumerate(chain):
+                st = steps[base + start + k]
+                if kind == "one":
+                    st.ones.append((site, t, op))
+                else:
+                    st.twos.append((site, t, op, fl))
+            for x in range(lo, hi + 1):
+                free[x] = start + len(chain)
+        ends.append(len(steps))
+    return steps, ends
+
+
+def bond_plan_steps(circuit: Circuit) -> List[List[int]]:
+    """Bond dimensions after every step of schedule(circuit): an update on
+    (i, i+1) sets bond i to k = min(d*chi_left, d*chi_right, maxsvals)
+    because the reference keeps zero singular values (TensorNetwork split
+    without an error cut)."""
     n, d = circuit.n, circuit.d
     bonds = [1] * (n + 1)  # bonds[0] and bonds[n] are the dummy end bonds
     out = []
-    for layer in circuit.layers:
-        for qudits, _ in layer:
-            if len(qudits) == 2:
-                i = min(qudits)
-                k = min(d * bonds[i], d * bonds[i + 2])
-                if circuit.maxsvals is not None:
-                    k = min(k, int(circuit.maxsvals))
-                bonds[i + 1] = k
+    steps, _ = schedule(circuit)
+    for st in steps:
+        for i, _, _, _ in st.twos:
+            k = min(d * bonds[i], d * bonds[i + 2])
+            if circuit.maxsvals is not None:
+                k = min(k, int(circuit.maxsvals))
+            bonds[i + 1] = k
         out.append(bonds[1:n])
     return [list(b) for b in out]
 
 
+def bond_plan(circuit: Circuit) -> List[List[int]]:
+    """Bond dimensions after every layer (bond_plan_steps at each layer's
+    last step), computed on the host from the circuit alone."""
+    per_step = bond_plan_steps(circuit)
+    _, ends = schedule(circuit)
+    first = [1] * (circuit.n - 1)
+    return [list(per_step[e - 1]) if e else list(first) for e in ends]
+
+
 def peak_footprint_bytes(circuit: Circuit) -> int:
-    """max over layers of sum_l chi_l * d * chi_{l+1} * 8 bytes (complex64)."""
+    """max over steps of sum_l chi_l * d * chi_{l+1} * 8 bytes (complex64)."""
     d = circuit.d
     best = circuit.n * d * 8
-    for b in bond_plan(circuit):
+    for b in bond_plan_steps(circuit):
         full = [1] + b + [1]
         tot = sum(full[i] * d * full[i + 1] for i in range(circuit.n)) * 8
         best = max(best, tot)
@@ -184,21 +274,21 @@ def two_site_bytes(chi_l: int, chi_m: int, chi_r: int, d: int, k: int) -> float:
 
 
 def circuit_work(circuit: Circuit) -> Tuple[float, float, int]:
-    """(flops, bytes, updates) of every two-site update of a circuit."""
+    """(flops, bytes, updates) of every two-site update of a circuit, SWAP
+    hops of non-adjacent ops included."""
     n, d = circuit.n, circuit.d
     bonds = [1] * (n + 1)
     fl = by = 0.0
     cnt = 0
-    for layer in circuit.layers:
-        for qudits, _ in layer:
-            if len(qudits) == 2:
-                i = min(qudits)
-                cl, cm, cr = bonds[i], bonds[i + 1], bonds[i + 2]
-                k = min(d * cl, d * cr)
-                if circuit.maxsvals is not None:
-                    k = min(k, int(circuit.maxsvals))
-                fl += two_site_flops(cl, cm, cr, d, k)
-                by += two_site_bytes(cl, cm, cr, d, k)
-                bonds[i + 1] = k
-                cnt += 1
+    steps, _ = schedule(circuit)
+    for st in steps:
+        for i, _, _, _ in st.twos:
+            cl, cm, cr = bonds[i], bonds[i + 1], bonds[i + 2]
+            k = min(d * cl, d * cr)
+            if circuit.maxsvals is not None:
+                k = min(k, int(circuit.maxsvals))
+            fl += two_site_flops(cl, cm, cr, d, k)
+            by += two_site_bytes(cl, cm, cr, d, k)
+            bonds[i + 1] = k
+            cnt += 1
     return fl, by, cnt

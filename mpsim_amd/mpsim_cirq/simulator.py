@@ -5,7 +5,15 @@ cirq.SimulatesFinalState; here the same methods are provided directly so
 that the simulator works with or without cirq installed. Accepted programs:
   - a cirq.Circuit (when cirq is importable): parameters are resolved,
     qubits ordered with QubitOrder.DEFAULT, ops converted by _unitary_();
-  - a mpsim_amd.circuits.Circuit (seeded synthetic workloads);
+  - any circuit object with all_qubits() and all_operations() (a cirq
+    circuit without cirq installed, or a stand-in): qubits in sorted order
+    (or qubit_order), ops converted as circuits.py:26-43 does;
+  - a mpsim_amd.circuits.Circuit (seeded synthetic workloads), which may
+    carry its own truncation cap (maxsvals) used when the options give none;
+    with params, a parameter sweep: each entry is a Circuit of the same
+    structure or the list of its op matrices, and all entries run as one
+    batched CircuitProgram (kept and reused, via update_gates, for later
+    sweeps of the same structure);
   - a sequence of MPSOperation, or of (qudit indices, matrix) pairs.
 The options dict is forwarded to MPS.apply as keyword arguments
 (simulator.py:86), e.g. {"maxsvals": 64} or {"fraction": 0.5}.
@@ -41,6 +49,7 @@ class MPSimulator:
     def __init__(self, options: Optional[dict] = None, device=None):
         self._options = {} if options is None else options
         self._device = device
+        self._programs: dict = {}  # structure -> CircuitProgram of parameter sweeps
 
     @property
     def options(self) -> dict:
@@ -54,6 +63,16 @@ class MPSimulator:
         """simulator.py:32-88: one MPS per parameter resolver."""
         if cirq is not None and isinstance(program, cirq.Circuit):
             return self._simulate_cirq(program, params, qubit_order)
+        if isinstance(program, Circuit) and params is not None:
+            return self._sweep(program, params)
+        if hasattr(program, "all_qubits") and hasattr(program, "all_operations"):
+            qubits = list(program.all_qubits())
+            ordered = list(qubit_order) if qubit_order is not None else sorted(qubits)
+            index = {q: i for i, q in enumerate(ordered)}
+            mps = MPS(nqudits=len(qubits), device=self._device)
+            mps.apply([mps_operation_from_gate_operation(op, index)
+                       for op in program.all_operations()], **self._options)
+            return [mps]
         if isinstance(program, Circuit):
             mps = MPS(program.n, program.d, device=self._device)
             opts = dict(self._options)
@@ -74,6 +93,47 @@ class MPSimulator:
         raise ValueError(
             f"Program is of type {type(program)} but should be a cirq.Circuit, a "
             "mpsim_amd.circuits.Circuit or a sequence of operations.")
+
+    def _sweep(self, program: Circuit, params) -> List[MPS]:
+        """One final state per entry of params (same structure as program,
+        new gate values), all run as one batched, captured schedule."""
+        variants = []
+        for p in params:
+            if isinstance(p, Circuit):
+                c = p
+            else:
+                mats = list(p)
+                if len(mats) != len(program.ops):
+                    raise ValueError(f"a parameter set has {len(mats)} matrices for "
+                                     f"{len(program.ops)} ops")
+                it = iter(mats)
+                c = Circuit(program.n, program.d, [[(q, next(it)) for q, _ in layer]
+                                                   for layer in program.layers],
+                            program.maxsvals, program.seed, program.name)
+            if [[q for q, _ in L] for L in c.layers] != [[q for q, _ in L] for L in program.layers] \
+                    or (c.n, c.d) != (program.n, program.d):
+                raise ValueError("every parameter set must have the program's structure")
+            variants.append(c)
+        if not variants:
+            return []
+        opts = dict(self._options)
+        if set(opts) - {"maxsvals"}:  # options the captured schedule does not take
+            return [self.simulate_sweep(c)[0] for c in variants]
+        cap = opts.get("maxsvals", program.maxsvals)
+        cap = None if cap is None else int(cap)
+        variants = [Circuit(c.n, c.d, c.layers, cap, c.seed, c.name) for c in variants]
+        from ..program import CircuitProgram
+        key = (program.n, program.d, cap, len(variants),
+               tuple(tuple(tuple(q) for q, _ in L) for L in program.layers))
+        prog = self._programs.get(key)
+        if prog is None:
+            from ..engine import Engine
+            prog = CircuitProgram(variants, Engine.get(self._device))
+            self._programs[key] = prog
+        else:
+            prog.update_gates(variants)
+        prog.run()
+        return [mps.copy() for mps in prog.states()]
 
     def _simulate_cirq(self, program, params, qubit_order) -> List[MPS]:  # pragma: no cover
         resolvers = cirq.to_resolvers(params) if params is not None else [cirq.ParamResolver({})]

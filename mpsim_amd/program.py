@@ -11,7 +11,7 @@ known before anything runs. A CircuitProgram fixes it once:
     factors back into the buffers of the two sites they read (the library
     reads the inputs into its workspace before it writes any output);
   * the descriptor tables of every tick (one grouped one-site call and one
-    or more grouped two-site calls per layer) are built on the host and
+    or more grouped two-site calls per step) are built on the host and
     copied to the device once;
   * the first run is eager with the usual per-sweep convergence poll, and
     records how many Jacobi sweeps each two-site call needed;
@@ -37,8 +37,8 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _native as nat
-from .batch import _ws_estimate, check_layer, upload_gates
-from .circuits import Circuit, bond_plan
+from .batch import _ws_estimate, op_matrix, upload_gates
+from .circuits import Circuit, bond_plan_steps, schedule, swap_matrix
 from .core import MPS
 from .engine import MAX_OPS, Engine, SvdConfig, TwoSiteItem
 
@@ -54,7 +54,8 @@ class _Call:
 
 
 class CircuitProgram:
-    """A batch of nearest-neighbour circuits compiled to one static schedule.
+    """A batch of circuits compiled to one static schedule (one grouped call
+    set per step of circuits.schedule, SWAP networks included).
 
     program = CircuitProgram(circuits)
     program.run()            # first run eager (records sweeps), then graph replays
@@ -75,8 +76,9 @@ class CircuitProgram:
         self.eager_runs = 0
         self.last_unconverged = 0
 
-        # -- site buffers ----------------------------------------------------
-        self._plans = [bond_plan(c) for c in self.circuits]
+        # -- site buffers (sized for the largest shape over every step) --------
+        self._steps = [schedule(c)[0] for c in self.circuits]
+        self._plans = [bond_plan_steps(c) for c in self.circuits]
         self._site_off: List[List[int]] = []
         total = 0
         for c, plan in zip(self.circuits, self._plans):
@@ -103,10 +105,11 @@ class CircuitProgram:
 
         # -- gates -------------------------------------------------------------
         self.gates = [upload_gates(eng, c) for c in self.circuits]
+        self._swaps = {c.d: eng.gate(swap_matrix(c.d)) for c in self.circuits}
 
-        # -- schedule ------------------------------------------------------------
-        depth = max((c.depth for c in self.circuits), default=0)
-        ntwo = sum(c.two_site_count() for c in self.circuits)
+        # -- schedule (one grouped call set per step, circuits.schedule) ---------
+        depth = max((len(st) for st in self._steps), default=0)
+        ntwo = sum(sum(len(x.twos) for x in st) for st in self._steps)
         self.discarded = torch.zeros(max(ntwo, 1), dtype=torch.float64, device=eng.dev)
         self.unconverged = torch.zeros(1, dtype=torch.int32, device=eng.dev)
         self.two_site_updates = ntwo
@@ -116,25 +119,24 @@ class CircuitProgram:
         for t in range(depth):
             one, two = [], []
             for ci, c in enumerate(self.circuits):
-                if t >= c.depth:
+                if t >= len(self._steps[ci]):
                     continue
                 d = c.d
                 before = [1] + (self._plans[ci][t - 1] if t else [1] * (c.n - 1)) + [1]
                 after = [1] + self._plans[ci][t] + [1]
                 offs = self._site_off[ci]
-                check_layer(c.layers[t], c.n, "a program")
-                for (q, _), g in zip(c.layers[t], self.gates[ci][t]):
-                    if len(q) == 1:
-                        s = q[0]
-                        p = self._addr(offs[s])
-                        one.append((p, p, nat.ptr(g), before[s], before[s + 1], d, 0))
-                    else:
-                        i = q[0]
-                        cl, cm, cr, k = before[i], before[i + 1], before[i + 2], after[i + 1]
-                        two.append(dict(a=self._addr(offs[i]), b=self._addr(offs[i + 1]),
-                                        gate=nat.ptr(g), cl=cl, cm=cm, cr=cr, d=d, k=k,
-                                        disc=disc_ptr + 8 * slot))
-                        slot += 1
+                step = self._steps[ci][t]
+                for site, lt, o in step.ones:
+                    p = self._addr(offs[site])
+                    one.append((p, p, nat.ptr(self.gates[ci][lt][o]), before[site], before[site + 1],
+                                d, 0))
+                for i, lt, o, _ in step.twos:
+                    g = self.gates[ci][lt][o] if o >= 0 else self._swaps[d]
+                    cl, cm, cr, k = before[i], before[i + 1], before[i + 2], after[i + 1]
+                    two.append(dict(a=self._addr(offs[i]), b=self._addr(offs[i + 1]),
+                                    gate=nat.ptr(g), cl=cl, cm=cm, cr=cr, d=d, k=k,
+                                    disc=disc_ptr + 8 * slot))
+                    slot += 1
             for s in range(0, len(one), MAX_OPS):
                 calls.append(_Call("one", np.array(one[s:s + MAX_OPS], dtype=nat.SITE_OP), 0))
             start = 0
@@ -280,8 +282,7 @@ class CircuitProgram:
                 raise ValueError(f"circuit {ci} has a different structure")
             for t, layer in enumerate(c.layers):
                 for (q, m), g in zip(layer, self.gates[ci][t]):
-                    g.copy_(self.engine.torch.from_numpy(
-                        np.ascontiguousarray(np.asarray(m, dtype=np.complex64))).view(g.shape))
+                    g.copy_(self.engine.torch.from_numpy(op_matrix(q, m, c.d)).view(g.shape))
         self.circuits = list(circuits)
 
     # ------------------------------------------------------------ results

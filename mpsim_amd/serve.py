@@ -5,7 +5,7 @@ exact device footprint fits the HBM budget and a slot is free. The footprint
 is known before the run because every bond dimension follows from the
 circuit alone (zero singular values are kept): the peak over layers of
 sum_l chi_l*d*chi_{l+1}*8 bytes for the site tensors, plus the Jacobi
-workspace of its largest two-site layer. Admitted circuits keep their gates
+workspace of its largest step of two-site updates. Admitted circuits keep their gates
 and tensors resident; a tick advances every active circuit of a lane by one
 layer in one grouped launch sequence (mpsim_amd.batch.tick), so requests
 that arrive while others run join at the lane's next tick and their work
@@ -34,23 +34,22 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from .batch import DeviceCircuit, prepare, tick, ws_bound
-from .circuits import Circuit, Request, bond_plan
+from .circuits import Circuit, Request, bond_plan_steps, schedule
 from .engine import Engine, SvdConfig
 
 
 def workspace_bytes(circuit: Circuit) -> int:
-    """Upper bound of the Jacobi workspace of the circuit's largest two-site
-    layer (batch.ws_bound per update)."""
+    """Upper bound of the Jacobi workspace of the circuit's largest step of
+    two-site updates (batch.ws_bound per update)."""
     d = circuit.d
     best = 0
     prev = [1] * (circuit.n + 1)
-    for layer, bonds in zip(circuit.layers, bond_plan(circuit)):
+    steps, _ = schedule(circuit)
+    for st, bonds in zip(steps, bond_plan_steps(circuit)):
         tot = 0
-        for q, _ in layer:
-            if len(q) == 2:
-                i = min(q)
-                M, N = d * prev[i], d * prev[i + 2]
-                tot += ws_bound(M, N)
+        for i, _, _, _ in st.twos:
+            M, N = d * prev[i], d * prev[i + 2]
+            tot += ws_bound(M, N)
         best = max(best, tot)
         prev = [1] + list(bonds) + [1]
     return best
@@ -59,7 +58,7 @@ def workspace_bytes(circuit: Circuit) -> int:
 def footprint_bytes(circuit: Circuit) -> int:
     d = circuit.d
     peak = circuit.n * d * 8
-    for b in bond_plan(circuit):
+    for b in bond_plan_steps(circuit):
         full = [1] + b + [1]
         peak = max(peak, 8 * sum(full[i] * d * full[i + 1] for i in range(circuit.n)))
     # old and new site tensors coexist during a layer

@@ -390,30 +390,36 @@ def test_gram_next_round_tiles_unshared():
         assert shared == (nb == 4), nb
 
 
-def test_layers_must_touch_each_qudit_once():
-    """A layer runs as one grouped one-site launch, then one grouped two-site
-    launch: overlapping ops in a layer (two pairs sharing a qudit, a one-site
-    gate on a qudit a pair also touches), non-adjacent pairs and qudits out
-    of range are rejected before anything runs."""
-    import torch
-    from mpsim_amd.batch import _layer_plan, check_layer
-    from mpsim_amd.circuits import Circuit
+def test_schedule_keeps_the_order_of_overlapping_ops():
+    """circuits.schedule: a layer of nearest-neighbour ops on disjoint qudits
+    is one step; a non-adjacent pair becomes the reference's SWAP network
+    (core.py:1035-1043, 1154-1219: hops right, the gate, hops back); ops
+    that share a site (two pairs on one qudit, a one-site gate inside a
+    network) run in list order; invalid ops raise before anything runs."""
+    from mpsim_amd.circuits import Circuit, bond_plan, schedule
     u2, u4 = np.eye(2), np.eye(4)
-    check_layer([((0, 1), u4), ((2, 3), u4), ((4,), u2)], 5)
-    bad = [
-        [((0, 1), u4), ((1, 2), u4)],          # pairs share qudit 1
-        [((0, 1), u4), ((1,), u2)],            # one-site gate on a pair's qudit
-        [((2,), u2), ((2,), u2)],              # the same qudit twice
-        [((0, 2), u4)],                        # not nearest-neighbour
-        [((3, 4), u4)],                        # out of range for 4 qudits
-    ]
-    for layer in bad:
+    steps, ends = schedule(Circuit(6, 2, [[((0, 1), u4), ((2, 3), u4), ((4,), u2)]]))
+    assert len(steps) == 1 and ends == [1]
+    assert steps[0].twos == [(0, 0, 0, False), (2, 0, 1, False)] and steps[0].ones == [(4, 0, 2)]
+    lay = [((0, 3), u4), ((4, 5), u4), ((1,), u2), ((1, 2), u4), ((5, 1), u4)]
+    steps, ends = schedule(Circuit(6, 2, [lay], 8))
+    twos = [[(i, o, f) for i, _, o, f in st.twos] for st in steps]
+    ones = [[(q, o) for q, _, o in st.ones] for st in steps]
+    assert twos[:5] == [[(0, -1, False), (4, 1, False)], [(1, -1, False)], [(2, 0, False)],
+                        [(1, -1, False)], [(0, -1, False)]]
+    assert ones[5] == [(1, 2)] and twos[5] == []
+    assert twos[6] == [(1, 3, False)]                       # (1, 2) after the one-site gate on 1
+    assert twos[7:] == [[(1, -1, False)], [(2, -1, False)], [(3, -1, False)], [(4, 4, True)],
+                        [(3, -1, False)], [(2, -1, False)], [(1, -1, False)]]
+    assert ends == [len(steps)]
+    # bond dimensions follow the sequential application (the oracle's rule)
+    from oracle.mps_oracle import OracleMPS
+    ora = OracleMPS(6, 2)
+    ora.apply_ops([(q, np.eye(2 ** len(q))) for q, _ in lay], maxsvals=8)
+    assert bond_plan(Circuit(6, 2, [lay], 8))[-1] == ora.bond_dimensions()
+    for bad in ([((0, 6), u4)], [((2, 2), u4)], [((0, 1, 2), np.eye(8))], [((-1,), u2)]):
         with pytest.raises(ValueError):
-            check_layer(layer, 4)
-        circ = Circuit(4, 2, [layer])
-        gates = [[torch.zeros(m.shape, dtype=torch.complex64) for _, m in layer]]
-        with pytest.raises(ValueError):
-            _layer_plan(circ, gates)
+            schedule(Circuit(6, 2, [bad]))
 
 
 def test_program_passes_the_svd_method():
@@ -432,3 +438,32 @@ def test_program_passes_the_svd_method():
     tab = _table([(64, 64, 64, 2, 64, 0)])
     rc, _, err = _plan(tab, CircuitProgram._opts(stub, 0))
     assert rc == 0 and int(tab[0]["method"]) == 0, err
+
+
+def test_gate_operation_conversion():
+    """circuits.py:26-43: a k-qubit operation with a unitary becomes an
+    MPSOperation on the mapped qudit indices whose tensor is the unitary
+    reshaped to [2]*2k; one without a unitary raises
+    CannotConvertToMPSOperation."""
+    from mpsim_amd.core import CannotConvertToMPSOperation
+    from mpsim_amd.mpsim_cirq.circuits import mps_operation_from_gate_operation
+
+    class Op:
+        def __init__(self, qubits, mat):
+            self.qubits, self._mat = tuple(qubits), mat
+
+        def _has_unitary_(self):
+            return self._mat is not None
+
+        def _unitary_(self):
+            return self._mat
+
+    u = np.arange(16).reshape(4, 4).astype(complex)
+    op = mps_operation_from_gate_operation(Op(["b", "a"], u), {"a": 0, "b": 3})
+    assert op.qudit_indices == (3, 0) and op.num_qudits == 2
+    assert op.tensor(reshape_to_square_matrix=False).shape == (2, 2, 2, 2)
+    assert np.array_equal(op.tensor(), u)
+    op1 = mps_operation_from_gate_operation(Op(["a"], np.eye(2)), {"a": 1})
+    assert op1.qudit_indices == (1,) and op1.is_valid()
+    with pytest.raises(CannotConvertToMPSOperation):
+        mps_operation_from_gate_operation(Op(["a"], None), {"a": 0})

@@ -75,3 +75,94 @@ def test_admission_respects_budget():
     assert rep.completed == 4 and rep.max_active == 2
     with pytest.raises(ValueError):
         MPSServer(hbm_budget=need // 2).run([Request(0, 0.0, circs[0])], realtime=False)
+
+
+def _nonlocal_circuit(seed, n, nlayers, cap):
+    """Random layers of one to three Haar two-qubit gates on arbitrary
+    (possibly distant, possibly reversed, possibly overlapping) pairs, then
+    a Haar U(2) on a random qubit."""
+    from mpsim_amd.circuits import Circuit
+    from mpsim_amd.gates import haar_unitary
+    rng = np.random.RandomState(seed)
+    layers = []
+    for _ in range(nlayers):
+        layer = []
+        for _ in range(rng.randint(1, 4)):
+            i, j = rng.choice(n, 2, replace=False)
+            layer.append(((int(i), int(j)), haar_unitary(4, rng)))
+        layer.append(((int(rng.randint(n)),), haar_unitary(2, rng)))
+        layers.append(layer)
+    return Circuit(n, 2, layers, cap, seed, f"nonlocal_s{seed}")
+
+
+def test_nonlocal_circuits_batched_and_captured():
+    """Non-adjacent pairs through the batched and captured paths: every op
+    on qudits (i, j) runs as the reference's SWAP network (hops, gate, hops
+    back, each hop a truncating two-site update; core.py:1035-1043,
+    1154-1219; pairs given high-to-low flipped as core.py:1031-1033). GHZ by
+    CNOT(0, k) for every k and a QFT's controlled phases (the reference's
+    core_test.py:1225-1258 circuits), and random non-local circuits with and
+    without truncation: the batched result is bitwise the plain API's (which
+    runs the same networks one gate at a time), the captured program's
+    replays are bitwise the batched result, and the states match the
+    complex128 oracle: overlap error <= 1e-6 where complex64 LAPACK itself
+    stays within 1e-8 of it, else within 3x complex64 LAPACK's error (the
+    10-qubit cap-8 circuit keeps exactly-zero singular values in some
+    updates, whose vectors are an arbitrary completion that later
+    truncations in the reference's non-canonical gauge may keep: complex64
+    LAPACK ends 0.14 from complex128 there); GHZ and QFT also against their
+    closed forms."""
+    import torch
+    from mpsim_amd.batch import prepare, run_batched, run_plain
+    from mpsim_amd.circuits import CNOT, Circuit
+    from mpsim_amd.program import CircuitProgram
+    from oracle.mps_oracle import OracleMPS, gauge_invariant_error
+    H = np.array([[1, 1], [1, -1]], dtype=np.complex128) / np.sqrt(2)
+    n = 7
+    ghz = Circuit(n, 2, [[((0,), H)], [((0, k), CNOT) for k in range(1, n)]], None, name="ghz")
+    qft_layers = []
+    for i in range(5):
+        qft_layers.append([((i,), H)])
+        qft_layers.append([((j, i), np.diag([1, 1, 1, np.exp(2j * np.pi / 2 ** (j - i + 1))]))
+                           for j in range(i + 1, 5)])
+    qft = Circuit(5, 2, [L for L in qft_layers if L], None, name="qft")
+    circs = [ghz, qft] + [_nonlocal_circuit(900 + s, m, 6, cap)
+                          for s, (m, cap) in enumerate([(6, None), (8, 4), (10, 8), (7, 16)])]
+    dcs = [prepare(c) for c in circs]
+    run_batched(dcs)
+    torch.cuda.synchronize()
+    checked = 0
+    for c, dc in zip(circs, dcs):
+        plain = run_plain(c)
+        assert _same(dc.mps.tensors, plain.tensors), c.name
+        kw = {} if c.maxsvals is None else {"maxsvals": c.maxsvals}
+        ora = OracleMPS(c.n, 2, dtype_mode="c128")
+        ora.apply_ops(c.ops, **kw)
+        c64 = OracleMPS(c.n, 2, dtype_mode="c64")
+        c64.apply_ops(c.ops, **kw)
+        assert dc.mps.bond_dimensions() == ora.bond_dimensions(), c.name
+        err, nr, nx = gauge_invariant_error(ora.tensors, [t.cpu().numpy() for t in dc.mps.tensors])
+        err64 = gauge_invariant_error(ora.tensors, c64.tensors)[0]
+        print(f"{c.name}: overlap error {err:.2e} (complex64 LAPACK {err64:.2e})")
+        if err64 <= 1e-8:
+            checked += 1
+            assert err <= 1e-6 and abs(nr - nx) <= 1e-6 * nr, (c.name, err)
+        else:
+            assert err <= 3 * err64, (c.name, err, err64)
+    assert checked >= 5
+    psi = dcs[0].mps.wavefunction()
+    want = np.zeros(2 ** n)
+    want[0] = want[-1] = 1 / np.sqrt(2)
+    assert np.allclose(psi, want, atol=1e-6)
+    # the QFT of |00000> is the uniform superposition
+    assert np.allclose(dcs[1].mps.wavefunction(), np.full(32, 1 / np.sqrt(32)), atol=1e-6)
+    prog = CircuitProgram(circs)
+    try:
+        for _ in range(3):
+            prog.run()
+            assert prog.last_unconverged == 0
+            for mps, dc in zip(prog.states(), dcs):
+                assert _same(mps.tensors, dc.mps.tensors)
+            torch.cuda.synchronize()
+    finally:
+        prog.close()
