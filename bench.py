@@ -187,15 +187,38 @@ def run_hbm(args, eng, svd):
     mps._tensors = [(torch.randn(bonds[i], 2, bonds[i + 1], dtype=torch.complex64, device=eng.dev)
                      / np.sqrt(2 * bonds[i])) for i in range(nq)]
     reps = 3
-    ms, nl = timed(lambda: mps._wavefunction_device(), "k_gemm", reps=reps)
-    by, rows = 0, 2
+    fn = lambda: mps._wavefunction_device()  # noqa: E731
+    fn()
+    torch.cuda.synchronize()
+    nat.profile_read(reset=True)
+    nat.profile_enable(True)
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    nat.profile_enable(False)
+    kk = nat.profile_read(reset=True)
+    ms = sum(kk.get(k, (0.0, 0))[0] for k in ("k_gemm", "k_gemm_rows"))
+    nl = sum(kk.get(k, (0.0, 0))[1] for k in ("k_gemm", "k_gemm_rows"))
+    # two-sided roofline per product: max(bytes / HBM peak, flops / FP32 vector
+    # peak without packed FP32, 78.6 TFLOP/s)
+    by, fl, bound, rows = 0, 0, 0.0, 2
     for i in range(1, nq):
         cl, cr = bonds[i], bonds[i + 1]
-        by += 8 * (rows * cl + cl * 2 * cr + rows * 2 * cr)
+        b_i = 8 * (rows * cl + cl * 2 * cr + rows * 2 * cr)
+        f_i = 8 * rows * cl * 2 * cr
+        by += b_i
+        fl += f_i
+        bound += max(b_i / (PEAK_HBM * 1e9), f_i / 78.6e12)
         rows *= 2
-    out["k_gemm_wavefunction"] = {"launches_per_call": nl // reps, "bytes_per_call": by,
-                                  "ms_per_call": round(ms / reps, 3),
-                                  "gb_per_s": round(by / (ms / reps * 1e-3) / 1e9, 1)}
+    t = ms / reps * 1e-3
+    out["k_gemm_wavefunction"] = {
+        "launches_per_call": nl // reps, "bytes_per_call": by, "flops_per_call": fl,
+        "ms_per_call": round(ms / reps, 3), "gb_per_s": round(by / t / 1e9, 1),
+        "tflops": round(fl / t / 1e12, 2),
+        "roofline_ms": round(bound * 1e3, 3), "frac_of_roofline": round(bound / t, 3),
+        "by_kernel_ms_per_call": {k: round(v[0] / reps, 3) for k, v in kk.items()},
+        "roofline_basis": "sum over the chain's products of max(bytes / 8 TB/s, flops / 78.6 "
+                          "TFLOP/s FP32 vector)"}
     del mps
 
     # theta and gather of 31 chi=256 updates

@@ -279,6 +279,12 @@ int mpsx_gemm(mpsx_stream stream, const mpsx_gemm_op *h, const mpsx_gemm_op *d, 
         max_n = std::max(max_n, o.n);
     }
     if (max_m == 0 || max_n == 0) return MPSX_OK;
+    bool rows = true;
+    for (int i = 0; i < count && rows; ++i) rows = gemm_rows_fit(h[i]);
+    if (rows) {
+        HIP_TRY(timed("k_gemm_rows", (hipStream_t)stream, [&] { return launch_gemm_rows((hipStream_t)stream, h, d, count); }), "k_gemm_rows");
+        return MPSX_OK;
+    }
     if (((max_m + 31) / 32) * ((max_n + 31) / 32) > 2147483647LL)
         return fail(MPSX_EINVAL, "m x n too large for one launch");
     HIP_TRY(timed("k_gemm", (hipStream_t)stream, [&] { return launch_gemm((hipStream_t)stream, d, count, max_m, max_n); }), "k_gemm");

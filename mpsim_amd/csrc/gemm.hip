@@ -81,6 +81,116 @@ __global__ void __launch_bounds__(256) k_gemm(const mpsx_gemm_op *ops, int64_t g
         }
 }
 
+// Tall, narrow products (no conjugation, k <= ROWS_KMAX, n <= 256): the
+// late steps of the wavefunction chain, m up to d^(n-1) rows with k and n of
+// a few. A workgroup takes a block of rows_per_block(k, n) consecutive rows:
+// it copies their k entries of A into LDS with 16-byte loads (one
+// contiguous, coalesced block when the rows are packed, lda = k), then
+// thread t computes column j = t % n of rows t / n, t / n + 256 / n, ...
+// from LDS (row-mates read the same entries, a broadcast) with column j of
+// B in registers; a wave's stores cover consecutive outputs. Bound by HBM
+// for small k and n.
+constexpr int ROWS_KMAX = 32;
+constexpr int ROWS_LDS_ELEMS = 8192;  // complex entries of A staged per workgroup (64 KiB)
+
+MPSX_HD int rows_per_block(int km, int n) {
+    const int per = 256 / n;
+    int passes = ROWS_LDS_ELEMS / (km * per);
+    passes = passes < 1 ? 1 : (passes > 8 ? 8 : passes);
+    return per * passes;
+}
+
+// KM: the op's k rounded up to a power of two (the unrolled length).
+template <int KM>
+__global__ void __launch_bounds__(256) k_gemm_rows(const mpsx_gemm_op *ops) {
+    extern __shared__ float2 As[];  // [rows of the block][KM]
+    const mpsx_gemm_op op = ops[blockIdx.y];
+    if (op.m <= 0 || op.n <= 0) return;
+    const int n = (int)op.n, k = (int)op.k, per = 256 / n;
+    const int rb = rows_per_block(KM, n);
+    const int64_t rbase = (int64_t)blockIdx.x * rb;
+    if (rbase >= op.m) return;
+    const int nrows = (int)((op.m - rbase) < rb ? (op.m - rbase) : rb);
+    const float2 *A = reinterpret_cast<const float2 *>(op.A);
+    const float2 *B = reinterpret_cast<const float2 *>(op.B);
+    float2 *C = reinterpret_cast<float2 *>(op.C);
+    const int t = threadIdx.x;
+    // stage A rows [rbase, rbase + nrows) as As[row][kk], kk < KM (zero-padded)
+    if (op.lda == k && (k & 1) == 0) {
+        const float4 *src = reinterpret_cast<const float4 *>(A + rbase * k);
+        const int pairs = nrows * k / 2;
+        for (int e = t; e < pairs; e += 256) {
+            const float4 v = src[e];
+            const int c = 2 * e, row = c / k, kk = c % k;
+            As[row * KM + kk] = make_float2(v.x, v.y);
+            As[row * KM + kk + 1] = make_float2(v.z, v.w);
+        }
+        if (k < KM)
+            for (int e = t; e < nrows * (KM - k); e += 256)
+                As[(e / (KM - k)) * KM + k + e % (KM - k)] = make_float2(0.f, 0.f);
+    } else {
+        for (int e = t; e < nrows * KM; e += 256) {
+            const int row = e / KM, kk = e % KM;
+            As[e] = kk < k ? A[(rbase + row) * op.lda + kk] : make_float2(0.f, 0.f);
+        }
+    }
+    __syncthreads();
+    if (t >= per * n) return;
+    const int j = t % n;
+    float2 bcol[KM];
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk)
+        bcol[kk] = kk < k ? B[(int64_t)kk * op.ldb + j] : make_float2(0.f, 0.f);
+    const float2 alpha = make_float2(op.alpha_re, op.alpha_im);
+    const bool accum = (op.flags & 4) != 0;
+    for (int row = t / n; row < nrows; row += per) {
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int kk = 0; kk < KM; ++kk) acc = cfma(As[row * KM + kk], bcol[kk], acc);
+        float2 v = cmul(alpha, acc);
+        float2 *dst = C + (rbase + row) * op.ldc + j;
+        if (accum) {
+            const float2 old = *dst;
+            v.x += old.x;
+            v.y += old.y;
+        }
+        *dst = v;
+    }
+}
+
+bool gemm_rows_fit(const mpsx_gemm_op &o) {
+    return (o.flags & 3) == 0 && o.k <= ROWS_KMAX && o.n >= 1 && o.n <= 256;
+}
+
+hipError_t launch_gemm_rows(hipStream_t s, const mpsx_gemm_op *h, const mpsx_gemm_op *d,
+                            int count) {
+    int64_t kmax = 1;
+    for (int i = 0; i < count; ++i) kmax = kmax > h[i].k ? kmax : h[i].k;
+    const int km = kmax <= 2 ? 2 : kmax <= 4 ? 4 : kmax <= 8 ? 8 : kmax <= 16 ? 16 : 32;
+    int64_t blocks = 1;
+    int rb_max = 1;
+    for (int i = 0; i < count; ++i) {
+        if (h[i].m <= 0 || h[i].n <= 0) continue;
+        const int rb = rows_per_block(km, (int)h[i].n);
+        rb_max = rb_max > rb ? rb_max : rb;
+        blocks = blocks > (h[i].m + rb - 1) / rb ? blocks : (h[i].m + rb - 1) / rb;
+    }
+    if (blocks > 2147483647LL || count > 65535) return hipErrorInvalidValue;
+    const dim3 g((unsigned)blocks, (unsigned)count), blk(256);
+    const size_t lds = sizeof(float2) * (size_t)rb_max * km;
+    if (km == 2)
+        hipLaunchKernelGGL(k_gemm_rows<2>, g, blk, lds, s, d);
+    else if (km == 4)
+        hipLaunchKernelGGL(k_gemm_rows<4>, g, blk, lds, s, d);
+    else if (km == 8)
+        hipLaunchKernelGGL(k_gemm_rows<8>, g, blk, lds, s, d);
+    else if (km == 16)
+        hipLaunchKernelGGL(k_gemm_rows<16>, g, blk, lds, s, d);
+    else
+        hipLaunchKernelGGL(k_gemm_rows<32>, g, blk, lds, s, d);
+    return hipGetLastError();
+}
+
 hipError_t launch_gemm(hipStream_t s, const mpsx_gemm_op *d, int count, int64_t max_m,
                        int64_t max_n) {
     const int64_t gm = (max_m + GT - 1) / GT, gn = (max_n + GT - 1) / GT;

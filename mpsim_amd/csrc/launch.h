@@ -11,6 +11,10 @@ hipError_t launch_one_site(hipStream_t s, const mpsx_site_op *d, int count, int6
 hipError_t launch_scale(hipStream_t s, const mpsx_scale_op *d, int count, int64_t max_n);
 hipError_t launch_gemm(hipStream_t s, const mpsx_gemm_op *d, int count, int64_t max_m,
                        int64_t max_n);
+// The tall-narrow variant: every op without conjugation, k <= 32, n <= 256.
+bool gemm_rows_fit(const mpsx_gemm_op &o);
+hipError_t launch_gemm_rows(hipStream_t s, const mpsx_gemm_op *h, const mpsx_gemm_op *d,
+                            int count);
 
 hipError_t launch_init_rows(hipStream_t s, bool f32, const mpsx_two_site_op *d, int count, char *ws,
                             int64_t rows);

@@ -282,3 +282,29 @@ def test_wavefunction_with_more_than_65535_row_tiles():
     psi = mps.wavefunction()
     assert psi.shape == (2 ** n,)
     assert np.max(np.abs(psi - 2.0 ** (-n / 2))) < 1e-6
+
+
+def test_tall_narrow_gemm_vs_torch():
+    """The tall-narrow GEMM path (no conjugation, k <= 32, n <= 256: the late
+    steps of the wavefunction chain) against a complex128 torch product,
+    with alpha and accumulation, for ragged m, k and n; and the tile path
+    (k = 33) on the same data."""
+    import torch
+    from mpsim_amd.engine import Engine
+    eng = Engine.get()
+    g = torch.Generator(device="cpu").manual_seed(5)
+    for m, k, n in ((1, 1, 1), (1000, 2, 2), (70001, 7, 5), (3000, 32, 64), (257, 16, 256),
+                    (513, 33, 3)):
+        A = torch.randn(m, k, dtype=torch.complex64, generator=g)
+        B = torch.randn(k, n, dtype=torch.complex64, generator=g)
+        C0 = torch.randn(m, n, dtype=torch.complex64, generator=g)
+        alpha = 0.75 - 0.5j
+        for accum in (False, True):
+            C = C0.clone().cuda()
+            eng.gemm([dict(A=A.cuda(), B=B.cuda(), C=C, m=m, n=n, k=k, lda=k, ldb=n, ldc=n,
+                           alpha=alpha, flags=4 if accum else 0)])
+            ref = alpha * (A.to(torch.complex128) @ B.to(torch.complex128))
+            if accum:
+                ref = ref + C0.to(torch.complex128)
+            err = (C.cpu().to(torch.complex128) - ref).abs().max().item()
+            assert err <= 1e-5 * max(1.0, ref.abs().max().item()) * max(1, k) ** 0.5, (m, k, n, err)
