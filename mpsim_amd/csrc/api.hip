@@ -509,7 +509,7 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
                 if (done) break;
             }
         }
-        HIP_TRY(timed("k_hxgemm", stream, [&] { return launch_hxgemm(stream, d, owner, ws, pairs, gram_c, 1); }), "k_hxgemm");
+        HIP_TRY(timed("k_hxgemm", stream, [&] { return launch_hxgemm(stream, d, owner, ws, pairs, gram_c, 1, gram_nb); }), "k_hxgemm");
         HIP_TRY(timed("k_hgram", stream, [&] { return launch_hgram(stream, d, owner, ws, pairs, gram_nb, 1); }), "k_hgram");
     }
     if (pairs > 0) {
@@ -569,7 +569,7 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
 
     if (has_gram) {
         HIP_TRY(timed("k_hxcopy", stream, [&] { return launch_hxcopy(stream, d, count, ws, rows); }), "k_hxcopy");
-        HIP_TRY(timed("k_hxgemm", stream, [&] { return launch_hxgemm(stream, d, owner, ws, pairs, gram_c, 0); }), "k_hxgemm");
+        HIP_TRY(timed("k_hxgemm", stream, [&] { return launch_hxgemm(stream, d, owner, ws, pairs, gram_c, 0, gram_nb); }), "k_hxgemm");
     }
     if (rows > 0) {
         HIP_TRY(timed("k_rownorm", stream, [&] { return launch_rownorm(stream, f32, d, count, ws, rows); }), "k_rownorm");

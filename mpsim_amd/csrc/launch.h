@@ -46,4 +46,4 @@ hipError_t launch_hround(hipStream_t s, bool f32h, const mpsx_two_site_op *d, co
                          int inner_sweeps, double abs_rel);
 hipError_t launch_hxcopy(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int64_t rows);
 hipError_t launch_hxgemm(hipStream_t s, const mpsx_two_site_op *d, const int *owner, char *ws,
-                         int64_t pairs, int ctiles, int mode);
+                         int64_t pairs, int ctiles, int mode, int max_nb);

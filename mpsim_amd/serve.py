@@ -131,8 +131,9 @@ class MPSServer:
     goes to the first lane whose cap is >= its maxsvals; maxsvals None counts
     as unbounded); () runs one lockstep lane for every request.
     lane_split_ways: how many concurrent stream groups a lane's large two-site
-    calls may use (Engine.split_ways), per lane; the default gives the last
-    (largest-bond) lane two and the others one, so the lanes use four streams.
+    calls may use (Engine.split_ways), per lane; by default (None) every run
+    spreads four streams (the hardware queues a process gets) over the lanes
+    that have requests: one each, the rest to the largest-bond one.
     """
 
     def __init__(self, engine: Optional[Engine] = None, hbm_budget: int = 64 << 30,
@@ -146,11 +147,10 @@ class MPSServer:
         self.keep_results = keep_results
         self.caps = tuple(int(c) for c in lanes)
         n = len(self.caps) + 1
-        if lane_split_ways is None:
-            lane_split_ways = [1] * (n - 1) + [2] if n > 1 else [self.engine.split_ways]
-        if len(lane_split_ways) != n:
+        if lane_split_ways is not None and len(lane_split_ways) != n:
             raise ValueError(f"lane_split_ways needs {n} entries")
-        self.split_ways = [int(w) for w in lane_split_ways]
+        self.fixed_ways = None if lane_split_ways is None else [int(w) for w in lane_split_ways]
+        self.split_ways = self.fixed_ways or [1] * n
         self._lane_engines: List[Engine] = []
 
     def lane_of(self, circuit: Circuit) -> int:
@@ -187,11 +187,19 @@ class MPSServer:
             need = footprint_bytes(r.circuit)
             if need > self.budget:
                 raise ValueError(f"request {r.rid} needs {need} B > budget {self.budget} B")
-        engines = self._engines()
-        n = len(engines)
+        n = len(self.caps) + 1
         queues: List[deque] = [deque() for _ in range(n)]
         for r in sorted(requests, key=lambda q: q.arrival):
             queues[self.lane_of(r.circuit)].append(r)
+        if self.fixed_ways is None:
+            busy = [i for i in range(n) if queues[i]]
+            ways = [1] * n
+            if n == 1:
+                ways = [self.engine.split_ways]
+            elif busy:
+                ways[busy[-1]] = max(1, 4 - (len(busy) - 1))
+            self.split_ways = ways
+        engines = self._engines()
         shared = _Shared(self.budget, self.max_active)
         stats = [dict(lane=i, cap=(self.caps[i] if i < len(self.caps) else None), requests=len(q),
                       ticks=0, max_active=0) for i, q in enumerate(queues)]
