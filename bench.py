@@ -221,12 +221,14 @@ def run_hbm(args, eng, svd):
                           "TFLOP/s FP32 vector)"}
     del mps
 
-    # theta and gather of 31 chi=256 updates
+    # theta and gather of 31 chi=256 updates, as one group (a split call would
+    # run four launches of each kernel concurrently, a quarter of the problems each)
     chi, cnt = 256, 31
     gate = eng.gate(haar_unitary(4, rng))
     its = [TwoSiteItem(torch.randn(chi, 2, chi, dtype=torch.complex64, device=eng.dev) / 32.0,
                        torch.randn(chi, 2, chi, dtype=torch.complex64, device=eng.dev) / 32.0,
                        gate, 2 * chi) for _ in range(cnt)]
+    ways, eng.split_ways = eng.split_ways, 1
     eng.two_site(its, svd)
     torch.cuda.synchronize()
     nat.profile_read(reset=True)
@@ -235,7 +237,9 @@ def run_hbm(args, eng, svd):
         eng.two_site(its, svd)
     torch.cuda.synchronize()
     nat.profile_enable(False)
+    eng.split_ways = ways
     k = nat.profile_read(reset=True)
+    assert k["k_theta"][1] == 2 and k["k_gather"][1] == 2, k
     zb = 8 if svd.fp32 else 16
     M = N = 2 * chi
     kk = 2 * chi  # the items keep every singular value

@@ -157,6 +157,10 @@ hipError_t timed(const char *name, hipStream_t s, F &&fn) {
 // or one per matrix row of a plain split.
 int64_t theta_tiles(const mpsx_two_site_op &o) {
     if (o.gate == nullptr) return o.chi_l;
+    if (o.d > 4) {  // generic kernel: one workgroup per (l, tile of r)
+        const int tr = theta_gen_tr(o.d);
+        return (int64_t)o.chi_l * (((int64_t)o.chi_r + tr - 1) / tr);
+    }
     return ((int64_t)o.chi_l + THETA_TL - 1) / THETA_TL * (((int64_t)o.chi_r + THETA_TR - 1) / THETA_TR);
 }
 
@@ -303,8 +307,9 @@ namespace {
 // anything is launched.
 int plan_shape(const mpsx_two_site_op &o, int i, const Resolved &r, mpsx_plan *p) {
     const bool split = o.gate == nullptr;
-    if (split ? o.d != 1 : (o.d < 2 || o.d > 4))
-        return fail(MPSX_EINVAL, "op %d: d=%d not in {2,3,4} (or 1 for a plain split)", i, o.d);
+    if (split ? o.d != 1 : (o.d < 2 || o.d > THETA_GEN_DMAX))
+        return fail(MPSX_EINVAL, "op %d: d=%d outside [2,%d] (or 1 for a plain split)", i, o.d,
+                    THETA_GEN_DMAX);
     if (o.chi_l < 0 || o.chi_m < 0 || o.chi_r < 0)
         return fail(MPSX_EINVAL, "op %d: negative bond dimension", i);
     const int64_t M = (int64_t)o.d * o.chi_l, N = (int64_t)o.d * o.chi_r;
@@ -404,7 +409,7 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
     // options, or was edited after planning is rejected.
     int64_t need = kHeaderBytes, tiles = 0, pairs = 0, rows = 0, outs = 0;
     int rows_nb = 0, gram_nb = 0, gram_y = 0, gram_c = 0;
-    bool has_d[5] = {false, false, false, false, false};  // index 1: plain splits
+    bool has_d[THETA_GEN_DMAX + 1] = {};  // index 1: plain splits
     bool has_b[17] = {false};  // one-sided problems by block width
     bool has_gram = false, has_mixed = false;
     for (int i = 0; i < count; ++i) {
@@ -456,6 +461,9 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
     if (rows > 0) HIP_TRY(timed("k_init_rows", stream, [&] { return launch_init_rows(stream, f32, d, count, ws, rows); }), "k_init_rows");
     for (int D = 1; D <= 4; ++D)
         if (has_d[D] && tiles > 0) HIP_TRY(timed("k_theta", stream, [&] { return launch_theta(stream, f32, D, d, count, ws, tiles); }), "k_theta");
+    bool has_gen = false;
+    for (int D = 5; D <= THETA_GEN_DMAX; ++D) has_gen = has_gen || has_d[D];
+    if (has_gen && tiles > 0) HIP_TRY(timed("k_theta", stream, [&] { return launch_theta_gen(stream, f32, d, count, ws, tiles); }), "k_theta");
     if (rows > 0) {
         HIP_TRY(timed("k_rownorm", stream, [&] { return launch_rownorm(stream, f32, d, count, ws, rows); }), "k_rownorm");
         HIP_TRY(timed("k_fro", stream, [&] { return launch_fro(stream, d, count, ws, r.null_rel); }), "k_fro");

@@ -161,3 +161,9 @@ MPSX_HD int64_t gather_units(int64_t M, int64_t k) {
 constexpr int THETA_TL = 32;  // l values per tile
 constexpr int THETA_TR = 32;  // r values per tile
 constexpr int THETA_KT = 16;  // bond (m) values per LDS stage
+
+// Qudit dimensions 5..16 take the generic theta kernel: one workgroup per
+// (l, tile of r values); the tile's d^2 x tr contraction T = A B stays in
+// LDS (at most 4096 complex64 entries) for the gate product.
+constexpr int THETA_GEN_DMAX = 16;
+MPSX_HD int theta_gen_tr(int d) { return d <= 8 ? 64 : 16; }

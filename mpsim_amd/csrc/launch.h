@@ -20,6 +20,8 @@ hipError_t launch_init_rows(hipStream_t s, bool f32, const mpsx_two_site_op *d, 
                             int64_t rows);
 hipError_t launch_theta(hipStream_t s, bool f32, int D, const mpsx_two_site_op *d, int count, char *ws,
                         int64_t tiles);
+hipError_t launch_theta_gen(hipStream_t s, bool f32, const mpsx_two_site_op *d, int count, char *ws,
+                            int64_t tiles);
 hipError_t launch_rownorm(hipStream_t s, bool f32, const mpsx_two_site_op *d, int count, char *ws,
                           int64_t rows);
 hipError_t launch_fro(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,

@@ -204,6 +204,64 @@ k_theta(const mpsx_two_site_op *ops, int count, char *ws) {
     }
 }
 
+// theta for a qudit dimension D in 5..16 (the kernels above are specialised
+// for D <= 4). One workgroup per (l, tile of tr r values), tr =
+// theta_gen_tr(D): first T[p1 p2][r] = sum_m A[l, p1, m] B[m, p2, r] for the
+// tile into LDS (FP32 accumulation, as k_theta), then
+// theta[(l, o1), (o2, r)] = sum_{p1 p2} G[(o1 o2), (p1 p2)] T[p1 p2][r].
+// The flops are those of the specialised kernel (d^2 chi_l chi_m chi_r for
+// T, d^4 chi_l chi_r for the gate); a wave's lanes take consecutive r, so
+// the reads of B and the writes of theta are coalesced and A is read once
+// per wave.
+template <class T>
+__global__ void __launch_bounds__(256)
+k_theta_gen(const mpsx_two_site_op *ops, int count, char *ws) {
+    __shared__ float2 Ts[4096];
+    const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
+                               OP_FIELD(plan.tile_base), blockIdx.x);
+    const mpsx_two_site_op &op = ops[pi];
+    const int D = op.d;
+    if (op.gate == nullptr || D <= 4 || D > THETA_GEN_DMAX) return;
+    const int chi_l = op.chi_l, chi_m = op.chi_m, chi_r = op.chi_r;
+    const mpsx_plan p = op.plan;
+    const int tr = theta_gen_tr(D), tiles_r = (chi_r + tr - 1) / tr;
+    const int64_t t = (int64_t)blockIdx.x - p.tile_base;
+    if (t >= (int64_t)chi_l * tiles_r) return;
+    const int l = (int)(t / tiles_r), r0 = (int)(t % tiles_r) * tr;
+    const float2 *A = reinterpret_cast<const float2 *>(op.a);
+    const float2 *B = reinterpret_cast<const float2 *>(op.b);
+    const float2 *G = reinterpret_cast<const float2 *>(op.gate);
+    const int n = D * D * tr;  // <= 4096
+    for (int e = threadIdx.x; e < n; e += 256) {
+        const int rr = e % tr, pp = e / tr, p1 = pp / D, p2 = pp % D, r = r0 + rr;
+        float2 acc = make_float2(0.f, 0.f);
+        if (r < chi_r) {
+            const float2 *a = A + ((int64_t)l * D + p1) * chi_m;
+            const float2 *bcol = B + (int64_t)p2 * chi_r + r;
+            for (int m = 0; m < chi_m; ++m) acc = cfma(a[m], bcol[(int64_t)m * D * chi_r], acc);
+        }
+        Ts[e] = acc;
+    }
+    __syncthreads();
+    typename Cx<T>::type *Z = z_ptr<T>(ws, p);
+    const int LD = p.LD;
+    const bool transposed = (op.flags & 1) != 0;
+    const int DD = D * D;
+    for (int e = threadIdx.x; e < n; e += 256) {
+        const int rr = e % tr, oo = e / tr, o1 = oo / D, o2 = oo % D, r = r0 + rr;
+        if (r >= chi_r) continue;
+        const float2 *g = G + (int64_t)oo * DD;
+        float2 v = make_float2(0.f, 0.f);
+        for (int pp = 0; pp < DD; ++pp) v = cfma(g[pp], Ts[pp * tr + rr], v);
+        const int64_t row = (int64_t)l * D + o1;      // theta row (l, o1)
+        const int64_t col = (int64_t)o2 * chi_r + r;  // theta col (o2, r)
+        if (!transposed)
+            Z[row * LD + col] = Cx<T>::make(v.x, v.y);
+        else
+            Z[col * LD + row] = Cx<T>::make(v.x, v.y);
+    }
+}
+
 // Plain split: one workgroup per row m of the M x N input matrix, copied
 // into X (or X^T for absorb-left problems).
 template <class T>
@@ -1885,6 +1943,15 @@ hipError_t launch_theta(hipStream_t s, bool f32, int D, const mpsx_two_site_op *
     const hipError_t e = f32 ? launch_theta_t<float>(s, D, d, count, ws, tiles)
                              : launch_theta_t<double>(s, D, d, count, ws, tiles);
     return e != hipSuccess ? e : hipGetLastError();
+}
+
+hipError_t launch_theta_gen(hipStream_t s, bool f32, const mpsx_two_site_op *d, int count, char *ws,
+                            int64_t tiles) {
+    if (f32)
+        hipLaunchKernelGGL(k_theta_gen<float>, dim3((unsigned)tiles), dim3(256), 0, s, d, count, ws);
+    else
+        hipLaunchKernelGGL(k_theta_gen<double>, dim3((unsigned)tiles), dim3(256), 0, s, d, count, ws);
+    return hipGetLastError();
 }
 
 hipError_t launch_rownorm(hipStream_t s, bool f32, const mpsx_two_site_op *d, int count, char *ws,

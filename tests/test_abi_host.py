@@ -106,12 +106,27 @@ def test_workspace_bound_covers_the_plan(cl, cr, d):
 
 @pytest.mark.parametrize("row,msg", [
     ((4, 4, 4, 2, 9, 0), "k=9"),
-    ((4, 4, 4, 5, 2, 0), "d=5"),
+    ((4, 4, 4, 17, 2, 0), "d=17"),
+    ((4, 4, 4, 0, 2, 0), "d=0"),
     ((-1, 4, 4, 2, 2, 0), "negative"),
 ])
 def test_plan_rejects_bad_descriptors(row, msg):
     rc, _, err = _plan(_table([row]))
     assert rc == 1 and msg in err
+
+
+def test_plan_generic_qudit_theta_tiles():
+    """Qudit dimensions 5..16 are planned (core.py:1013-1028 accepts any d
+    whose gate edges match the MPS); their theta runs one workgroup per
+    (l, tile of r values), 64 r values for d <= 8 and 16 above, so the
+    theta workgroup prefix advances by chi_l * ceil(chi_r / tile)."""
+    rows = [(7, 5, 70, 5, 35, 0), (3, 9, 20, 9, 27, 1), (2, 2, 2, 16, 32, 0), (33, 4, 40, 2, 66, 0)]
+    tab = _table(rows)
+    rc, _, err = _plan(tab)
+    assert rc == 0, err
+    tiles = [7 * 2, 3 * 2, 2 * 1, 2 * 2]
+    assert list(tab["tile_base"]) == [0, tiles[0], tiles[0] + tiles[1], sum(tiles[:3])]
+    assert (tab[0]["C"], tab[0]["R"]) == (35, 350) and (tab[1]["C"], tab[1]["R"]) == (180, 27)
 
 
 def test_plan_rejects_bad_options():

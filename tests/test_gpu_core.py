@@ -392,6 +392,53 @@ def test_qutrit_truncated_vs_fp64_oracle(left):
           f"worst kept-value error {_check_spectra(rec, ora):.2e}")
 
 
+@pytest.mark.parametrize("d,n,depth", [(5, 4, 4), (6, 3, 3), (8, 3, 3), (16, 3, 2)])
+def test_large_qudit_untruncated_vs_statevector(d, n, depth):
+    """Qudit dimensions above 4 (core.py:1013-1028 accepts any d whose gate
+    edges match the MPS; here the generic theta kernel): untruncated d^2 x d^2
+    Haar brickwork against the FP64 state vector (<= 1e-5). A gate on a
+    non-adjacent pair raises as in the reference, whose SWAP network applies
+    the qubit SWAP (core.py:1188-1189, "SWAP is only for qubits")."""
+    m = _mpsim()
+    from mpsim_amd.circuits import brickwork_haar
+    from mpsim_amd.gates import haar_unitary
+    circ = brickwork_haar(n, depth, d, maxsvals=None, seed=50 + d)
+    mps = m.MPS(n, d)
+    for qudits, mat in circ.ops:
+        mps.apply_two_qudit_gate(mat.reshape(d, d, d, d), *qudits)
+    ref = sv.run(circ.ops, n, d)
+    err = np.max(np.abs(mps.wavefunction() - ref))
+    print(f"d={d} n={n}: max error vs FP64 state vector {err:.2e}")
+    assert err <= 1e-5, err
+    far = haar_unitary(d * d, np.random.RandomState(d)).reshape(d, d, d, d)
+    with pytest.raises(ValueError, match="should have MPS qudit dimension"):
+        mps.apply_two_qudit_gate(far, 0, n - 1)
+
+
+@pytest.mark.parametrize("left", [True, False])
+def test_large_qudit_truncated_vs_fp64_oracle(left):
+    """d = 5 (generic theta kernel), 6 qudits, depth 6, maxsvals 12: bond
+    dimensions equal to the FP64 oracle's, gauge-invariant overlap error and
+    norm within 1e-4, and every update's kept singular values and discarded
+    weight against the oracle's split (_check_spectra)."""
+    m = _mpsim()
+    from mpsim_amd.circuits import brickwork_haar
+    from oracle.mps_oracle import gauge_invariant_error
+    d, cap = 5, 12
+    circ = brickwork_haar(6, 6, d, maxsvals=cap, seed=55)
+    ora = OracleMPS(6, d, dtype_mode="c128")
+    for qudits, mat in circ.ops:
+        ora.apply_two_qudit_gate(mat.reshape(d, d, d, d), *qudits, maxsvals=cap,
+                                 keep_left_canonical=left)
+    mps, rec = _recorded_updates(m, circ, cap, left)
+    assert mps.bond_dimensions() == ora.bond_dimensions()
+    err, nr, nx = gauge_invariant_error(ora.tensors, [t.cpu().numpy() for t in mps.tensors])
+    assert err <= 1e-4, err
+    assert abs(nr - nx) <= 1e-4 * nr
+    print(f"d=5 left={left}: overlap error {err:.2e}, "
+          f"worst kept-value error {_check_spectra(rec, ora):.2e}")
+
+
 def test_singular_values_of_one_update_vs_lapack():
     """Kept singular values of a single two-site update at a cap-size shape
     (theta 512 x 512 -> k = 256) against numpy's FP64 SVD: <= 1e-4 relative
