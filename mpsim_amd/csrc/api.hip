@@ -228,7 +228,7 @@ int mpsx_apply_one_site(mpsx_stream stream, const mpsx_site_op *h, const mpsx_si
     if (!h || !d) return fail(MPSX_EINVAL, "null descriptor table");
     if (count > 65535) return fail(MPSX_EINVAL, "count %d > 65535", count);
     int64_t max_pairs = 0;
-    int max_d = 0;
+    uint32_t dmask = 0;  // bit d: some op has qudit dimension d
     for (int i = 0; i < count; ++i) {
         const mpsx_site_op &o = h[i];
         if (o.d < 1 || o.d > 16) return fail(MPSX_EINVAL, "op %d: d=%d outside [1,16]", i, o.d);
@@ -237,10 +237,10 @@ int mpsx_apply_one_site(mpsx_stream stream, const mpsx_site_op *h, const mpsx_si
         if (pairs > 0 && (!o.a || !o.out || !o.gate))
             return fail(MPSX_EINVAL, "op %d: null pointer", i);
         max_pairs = std::max(max_pairs, pairs);
-        max_d = std::max(max_d, o.d);
+        if (pairs > 0) dmask |= 1u << o.d;
     }
     if (max_pairs == 0) return MPSX_OK;
-    HIP_TRY(timed("k_one_site", (hipStream_t)stream, [&] { return launch_one_site((hipStream_t)stream, d, count, max_pairs, max_d); }), "k_one_site");
+    HIP_TRY(timed("k_one_site", (hipStream_t)stream, [&] { return launch_one_site((hipStream_t)stream, d, count, max_pairs, dmask); }), "k_one_site");
     return MPSX_OK;
 }
 

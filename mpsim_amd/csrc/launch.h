@@ -7,7 +7,7 @@
 #include "../../include/mpsx.h"
 
 hipError_t launch_one_site(hipStream_t s, const mpsx_site_op *d, int count, int64_t max_pairs,
-                           int max_d);
+                           uint32_t dmask);
 hipError_t launch_scale(hipStream_t s, const mpsx_scale_op *d, int count, int64_t max_n);
 hipError_t launch_gemm(hipStream_t s, const mpsx_gemm_op *d, int count, int64_t max_m,
                        int64_t max_n);

@@ -47,23 +47,37 @@ MPSX_DEV void one_site_generic(const mpsx_site_op &op, const float2 *g, int64_t 
     }
 }
 
+// One kernel per qudit dimension 2..4 (ops of other dimensions return at
+// once), launched for the dimensions a call has.
+template <int D>
 __global__ void __launch_bounds__(256) k_one_site(const mpsx_site_op *ops) {
+    __shared__ float2 g[D * D];
+    const mpsx_site_op op = ops[blockIdx.y];
+    if (op.d != D) return;
+    const float2 *G = reinterpret_cast<const float2 *>(op.gate);
+    for (int i = threadIdx.x; i < D * D; i += blockDim.x) g[i] = G[i];
+    __syncthreads();
+    const int64_t pairs = op.chi_l * op.chi_r;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < pairs;
+         e += (int64_t)gridDim.x * blockDim.x)
+        one_site_body<D>(op, g, e);
+}
+
+// Every other dimension (1, and 5..16). Kept apart from the unrolled kernels
+// because its runtime-indexed input array lives in scratch memory, which
+// every launch of a kernel that has it must reserve.
+__global__ void __launch_bounds__(256) k_one_site_gen(const mpsx_site_op *ops) {
     __shared__ float2 g[SITE_MAXD * SITE_MAXD];
     const mpsx_site_op op = ops[blockIdx.y];
     const int d = op.d;
+    if (d >= 2 && d <= 4) return;
     const float2 *G = reinterpret_cast<const float2 *>(op.gate);
     for (int i = threadIdx.x; i < d * d; i += blockDim.x) g[i] = G[i];
     __syncthreads();
     const int64_t pairs = op.chi_l * op.chi_r;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < pairs;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        switch (d) {
-            case 2: one_site_body<2>(op, g, e); break;
-            case 3: one_site_body<3>(op, g, e); break;
-            case 4: one_site_body<4>(op, g, e); break;
-            default: one_site_generic(op, g, e); break;
-        }
-    }
+         e += (int64_t)gridDim.x * blockDim.x)
+        one_site_generic(op, g, e);
 }
 
 __global__ void __launch_bounds__(256) k_scale(const mpsx_scale_op *ops) {
@@ -86,10 +100,13 @@ static unsigned blocks_for(int64_t n, int64_t per_block, unsigned cap) {
 }
 
 hipError_t launch_one_site(hipStream_t s, const mpsx_site_op *d, int count, int64_t max_pairs,
-                           int max_d) {
-    (void)max_d;
+                           uint32_t dmask) {
     dim3 grid(blocks_for(max_pairs, 256, 8192), (unsigned)count);
-    hipLaunchKernelGGL(k_one_site, grid, dim3(256), 0, s, d);
+    if (dmask & (1u << 2)) hipLaunchKernelGGL(k_one_site<2>, grid, dim3(256), 0, s, d);
+    if (dmask & (1u << 3)) hipLaunchKernelGGL(k_one_site<3>, grid, dim3(256), 0, s, d);
+    if (dmask & (1u << 4)) hipLaunchKernelGGL(k_one_site<4>, grid, dim3(256), 0, s, d);
+    if (dmask & ~((1u << 2) | (1u << 3) | (1u << 4)))
+        hipLaunchKernelGGL(k_one_site_gen, grid, dim3(256), 0, s, d);
     return hipGetLastError();
 }
 

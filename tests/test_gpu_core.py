@@ -396,17 +396,23 @@ def test_qutrit_truncated_vs_fp64_oracle(left):
 def test_large_qudit_untruncated_vs_statevector(d, n, depth):
     """Qudit dimensions above 4 (core.py:1013-1028 accepts any d whose gate
     edges match the MPS; here the generic theta kernel): untruncated d^2 x d^2
-    Haar brickwork against the FP64 state vector (<= 1e-5). A gate on a
+    Haar brickwork after a d x d Haar gate on every qudit (the generic
+    one-site kernel) against the FP64 state vector (<= 1e-5). A gate on a
     non-adjacent pair raises as in the reference, whose SWAP network applies
     the qubit SWAP (core.py:1188-1189, "SWAP is only for qubits")."""
     m = _mpsim()
     from mpsim_amd.circuits import brickwork_haar
     from mpsim_amd.gates import haar_unitary
     circ = brickwork_haar(n, depth, d, maxsvals=None, seed=50 + d)
+    rng = np.random.RandomState(60 + d)
+    ops = [((q,), haar_unitary(d, rng)) for q in range(n)] + list(circ.ops)
     mps = m.MPS(n, d)
-    for qudits, mat in circ.ops:
-        mps.apply_two_qudit_gate(mat.reshape(d, d, d, d), *qudits)
-    ref = sv.run(circ.ops, n, d)
+    for qudits, mat in ops:
+        if len(qudits) == 1:
+            mps.apply_one_qudit_gate(mat, qudits[0])
+        else:
+            mps.apply_two_qudit_gate(mat.reshape(d, d, d, d), *qudits)
+    ref = sv.run(ops, n, d)
     err = np.max(np.abs(mps.wavefunction() - ref))
     print(f"d={d} n={n}: max error vs FP64 state vector {err:.2e}")
     assert err <= 1e-5, err
