@@ -33,7 +33,7 @@ def _matrix_addr(x, rows: int, cols: int, ld: int, what: str) -> int:
     return nat.ptr(t) + 8 * off
 
 
-SPLIT_ROWS = 128  # a problem this size keeps a round busy long enough to overlap
+SPLIT_ROWS = 128  # default split_rows: rows from which a problem counts toward splitting a call
 
 
 def _rows(it) -> int:
@@ -119,11 +119,14 @@ class Engine:
         self.dev = nat.device(dev)
         self._ws = None
         self._gates: "OrderedDict[bytes, object]" = OrderedDict()
-        from .tune import tuned_config
+        from .tune import tuned_config, tuned_engine
         self.svd = tuned_config()
         self.last_sweeps = 0
-        self.split_min = 8          # problems with >= SPLIT_ROWS rows needed to split a call
-        self.split_ways = 4         # concurrent groups of a split call (4 hardware queues)
+        # concurrent groups of a split call (at most the 4 hardware queues a
+        # process gets), the problems of >= split_rows rows a call needs to
+        # be split, and that row count; from the stored tuning when there is
+        # one (SPLIT_ROWS and 4 ways otherwise)
+        self.split_ways, self.split_min, self.split_rows = tuned_engine()
         self._wss: list = []        # workspaces of the helper streams' groups
         self._helpers: list = []    # (stream, single-thread executor) per helper group
         self._keep: list = []       # device tables of the last call, alive until the next
@@ -275,7 +278,7 @@ class Engine:
     def two_site(self, items: Sequence[TwoSiteItem], svd: Optional[SvdConfig] = None
                  ) -> List[TwoSiteResult]:
         """Batched two-site updates on the current stream. A call with at
-        least `split_min` problems of SPLIT_ROWS or more rows runs as
+        least `split_min` problems of `split_rows` or more rows runs as
         `split_ways` groups of equal work concurrently, group 0 on the current
         stream and each other group from its own worker thread on its own
         helper stream: the latency-bound pivot eigensolves of one group then
@@ -286,7 +289,7 @@ class Engine:
             return []
         cfg = (svd or self.svd).to_c()
         self._keep = []
-        big = [i for i, it in enumerate(items) if _rows(it) >= SPLIT_ROWS]
+        big = [i for i, it in enumerate(items) if _rows(it) >= self.split_rows]
         ways = max(1, min(int(self.split_ways), n))
         if ways < 2 or len(big) < self.split_min or cfg.check_every == 0:
             outs, tab, dtab, ws = self._prepare_two_site(items, cfg)
