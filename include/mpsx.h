@@ -130,7 +130,10 @@ typedef struct {
     void *b_out;       /* [k, d, chi_r]                                  */
     float *s_out;      /* k singular values, or NULL                     */
     double *discarded; /* sum of squared discarded values, or NULL       */
-    int32_t chi_l, chi_m, chi_r, d;
+    int32_t chi_l, chi_m, chi_r, d;  /* d: qudit dimension 2..16 (core.py:1013-1028
+                          accepts any d matching the MPS), or 1 with gate
+                          = NULL for a plain split of the chi_l x chi_r
+                          matrix a                                        */
     int32_t k;
     int32_t flags;     /* bit0: absorb S into the left factor            */
     mpsx_plan plan;    /* library-owned, see mpsx_two_site_plan          */
