@@ -95,18 +95,33 @@ class ServeReport:
 
 
 class _Shared:
-    """State the lanes share: the HBM budget, the active count and the
-    results (one lock)."""
+    """State the lanes share: the HBM budget, the active count, the lanes
+    that still have work and the results (one lock)."""
 
-    def __init__(self, budget: int, max_active: int) -> None:
+    def __init__(self, budget: int, max_active: int, busy: Sequence[int] = ()) -> None:
         self.lock = threading.Lock()
         self.budget = budget
         self.used = 0
         self.max_active = max_active
         self.active = 0
         self.peak_active = 0
+        self.busy = set(busy)
         self.done: List[Served] = []
         self.error: Optional[BaseException] = None
+
+    def lane_done(self, i: int) -> None:
+        with self.lock:
+            self.busy.discard(i)
+
+    def ways(self, i: int, queues: int = 4) -> int:
+        """Stream groups for lane i's split calls: the hardware queues spread
+        over the lanes that still have work, one each and the rest to the
+        largest-bond one."""
+        with self.lock:
+            busy = sorted(self.busy)
+        if not busy or i != busy[-1]:
+            return 1
+        return max(1, queues - (len(busy) - 1))
 
     def try_admit(self, need: int) -> bool:
         with self.lock:
@@ -131,9 +146,10 @@ class MPSServer:
     goes to the first lane whose cap is >= its maxsvals; maxsvals None counts
     as unbounded); () runs one lockstep lane for every request.
     lane_split_ways: how many concurrent stream groups a lane's large two-site
-    calls may use (Engine.split_ways), per lane; by default (None) every run
-    spreads four streams (the hardware queues a process gets) over the lanes
-    that have requests: one each, the rest to the largest-bond one.
+    calls may use (Engine.split_ways), per lane; by default (None) the four
+    streams (the hardware queues a process gets) are spread, at every tick,
+    over the lanes that still have requests: one each, the rest to the
+    largest-bond one (so a lane left alone gets all four).
     """
 
     def __init__(self, engine: Optional[Engine] = None, hbm_budget: int = 64 << 30,
@@ -191,8 +207,8 @@ class MPSServer:
         queues: List[deque] = [deque() for _ in range(n)]
         for r in sorted(requests, key=lambda q: q.arrival):
             queues[self.lane_of(r.circuit)].append(r)
+        busy = [i for i in range(n) if queues[i]]
         if self.fixed_ways is None:
-            busy = [i for i in range(n) if queues[i]]
             ways = [1] * n
             if n == 1:
                 ways = [self.engine.split_ways]
@@ -200,7 +216,7 @@ class MPSServer:
                 ways[busy[-1]] = max(1, 4 - (len(busy) - 1))
             self.split_ways = ways
         engines = self._engines()
-        shared = _Shared(self.budget, self.max_active)
+        shared = _Shared(self.budget, self.max_active, busy)
         stats = [dict(lane=i, cap=(self.caps[i] if i < len(self.caps) else None), requests=len(q),
                       ticks=0, max_active=0) for i, q in enumerate(queues)]
         # every earlier allocation is complete before the lane streams start
@@ -214,6 +230,8 @@ class MPSServer:
             except BaseException as e:  # re-raised in the calling thread
                 with shared.lock:
                     shared.error = shared.error or e
+            finally:
+                shared.lane_done(i)
 
         if n == 1:
             lane(0)
@@ -273,6 +291,8 @@ class MPSServer:
                     wait = pending[0].arrival - now() if realtime else 0.0
                     time.sleep(max(0.0005, min(wait, 0.01)))
                 continue
+            if self.fixed_ways is None and len(self.caps) > 0:
+                eng.split_ways = shared.ways(i)
             tick(eng, [a[0] for a in active], self.svd)
             stats["ticks"] += 1
             finished = [a for a in active if a[0].done]

@@ -65,3 +65,17 @@ def test_tuner_turns_splitting_off_when_it_never_pays(tmp_path):
                     conc_chis=(32, 64), layer=4, timer=timer)
     assert out["split_ways"] == 1 and out["split_rows"] == tune.NO_SPLIT_ROWS
     assert "split_min" not in {r["stage"] for r in out["measurements"]}
+
+
+def test_serving_lane_ways_follow_busy_lanes():
+    """The four hardware queues go to the lanes that still have work: one
+    each, the rest to the largest-bond lane, re-spread as lanes finish."""
+    from mpsim_amd.serve import _Shared
+    s = _Shared(1, 1, busy=[0, 1, 2])
+    assert [s.ways(i) for i in range(3)] == [1, 1, 2]
+    s.lane_done(2)
+    assert [s.ways(i) for i in range(2)] == [1, 3]
+    s.lane_done(0)
+    assert s.ways(1) == 4
+    s.lane_done(1)
+    assert s.ways(1) == 1
