@@ -1,18 +1,21 @@
-"""Single-GPU serving loop with continuous, layer-level batching.
+"""Single-GPU serving loop with continuous, step-level batching.
 
 Requests (seeded circuits with an arrival time) are admitted while their
 exact device footprint fits the HBM budget and a slot is free. The footprint
 is known before the run because every bond dimension follows from the
-circuit alone (zero singular values are kept): the peak over layers of
-sum_l chi_l*d*chi_{l+1}*8 bytes for the site tensors, plus the Jacobi
-workspace of its largest step of two-site updates. Admitted circuits keep their gates
-and tensors resident; a tick advances every active circuit of a lane by one
-layer in one grouped launch sequence (mpsim_amd.batch.tick), so requests
-that arrive while others run join at the lane's next tick and their work
-overlaps on the device. Finished requests free their budget immediately.
+circuit alone (zero singular values are kept): the peak over the steps of
+its schedule (circuits.schedule: disjoint nearest-neighbour pairs, SWAP
+networks for non-adjacent ones) of sum_l chi_l*d*chi_{l+1}*8 bytes for the
+site tensors, plus the Jacobi workspace of its largest step of two-site
+updates. Admitted circuits keep their gates and tensors resident; a tick
+advances every active circuit of a lane by one step (one layer of a
+brickwork circuit) in one grouped launch sequence (mpsim_amd.batch.tick),
+so requests that arrive while others run join at the lane's next tick and
+their work overlaps on the device. Finished requests free their budget
+immediately.
 
 Lanes. A tick lasts as long as its slowest two-site update, so one lockstep
-loop over every request makes a 64-bond circuit wait for each layer of a
+loop over every request makes a 64-bond circuit wait for each step of a
 512-bond one. Requests are therefore split by bond cap into lanes (by
 default caps <= 128, <= 256 and the rest); each lane has its own stream,
 engine (workspace and helper streams) and host thread, admits from its own
@@ -61,7 +64,7 @@ def footprint_bytes(circuit: Circuit) -> int:
     for b in bond_plan_steps(circuit):
         full = [1] + b + [1]
         peak = max(peak, 8 * sum(full[i] * d * full[i + 1] for i in range(circuit.n)))
-    # old and new site tensors coexist during a layer
+    # old and new site tensors coexist during a step
     return 2 * peak + workspace_bytes(circuit)
 
 
