@@ -296,6 +296,19 @@ def run_qutrit(args, eng, svd):
     line["extra"] = {"kernel_ms": {k: round(v[0], 2) for k, v in sorted(kern.items())},
                      "bond_dimensions_max": max(dc.mps.bond_dimensions()),
                      "algorithmic_tflops_per_s": round(flops / dt / 1e12, 3)}
+    if not args.no_cpu:
+        # the reference-faithful restatement on the host: one update at the cap
+        # shape (theta 3072 x 3072 -> 1024, complex128, LAPACK SVD, the per-gate
+        # full norm of the 48-site MPS)
+        from oracle.cpu_baseline import blas_threads, cpu_model, time_updates
+        r = time_updates(updates=1, n=48, d=3, cap=1024, mode="reference", track_norms=True)
+        line["cpu_baseline"] = {
+            "value": round(r["updates_per_s"], 5), "unit": "2-site updates/s", "cores": blas_threads(),
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": "one two-site update at the cap shape (site tensors [1024, 3, 1024], theta "
+                      "3072 x 3072 -> 1024) of the 48-qutrit chain, reference-faithful restatement "
+                      "(complex128, np.linalg.svd, S absorbed right, per-gate full norm)",
+            "seconds": round(r["seconds"], 2)}
     emit(line)
 
 
