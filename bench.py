@@ -8,8 +8,11 @@ Jacobi SVD launch sequence. Gate matrices are resident in HBM before the
 timed region. value = updates / second over the K timed steps.
 
 Also reported: the plain one-gate-at-a-time GPU path on the same circuit,
-the dominant kernel's roofline (HIP events on its launch stream), and the
-reference-faithful CPU restatement timed on this host on a bounded sample.
+the dominant kernel's roofline (HIP events on its launch stream), the
+reference-faithful CPU restatement timed on this host on a bounded sample,
+the served stream (`served`: continuous batching, p50/p99, vs the plain
+path) and the uneven batch of 512 circuits (`uneven`: eager step ticks, one
+captured hipGraph, vs the plain path).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain]
 """
@@ -60,6 +63,8 @@ def parse():
     ap.add_argument("--requests", type=int, default=40,
                     help="served leg: first N requests of the 200 req/s stream")
     ap.add_argument("--no-served", action="store_true", help="skip the served leg of the metric line")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven-batch object of the metric line")
     ap.add_argument("--lanes", default="128,256",
                     help="served: bond caps bounding the serving lanes ('' = one lockstep lane)")
     ap.add_argument("--rate", type=float, default=200.0)
@@ -312,22 +317,25 @@ def run_qutrit(args, eng, svd):
     emit(line)
 
 
-def run_uneven(args, eng, svd):
-    """Batched uneven circuits: grouped layer ticks over the whole batch vs the
-    plain path (one circuit, one gate per call) on a sample."""
+def uneven_leg(args, eng, svd, steps: int, warmup: int):
+    """The uneven-batch configuration (512 seeded circuits of n in [12,64],
+    depth in [4,24], bond caps in {32,64,128,256}): grouped step ticks over the
+    whole batch (eager), the same batch as one captured CircuitProgram (first
+    run eager, then hipGraph replays), and the plain path (one circuit, one
+    gate per call) on a sample. Returns (eager seconds per batch, extra)."""
     import torch
     from mpsim_amd.batch import prepare, reset, run_batched, run_plain
     from mpsim_amd.circuits import uneven_batch
     circs = uneven_batch(args.circuits)
     dcs = [prepare(c, eng) for c in circs]
     updates = sum(c.two_site_count() for c in circs)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         for dc in dcs:
             reset(dc)
         run_batched(dcs, eng, svd)
     torch.cuda.synchronize()
     times = []
-    for _ in range(args.steps):
+    for _ in range(steps):
         for dc in dcs:
             reset(dc)
         torch.cuda.synchronize()
@@ -336,15 +344,14 @@ def run_uneven(args, eng, svd):
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
     tb = sum(times) / len(times)
-    # the same batch as one captured schedule (CircuitProgram: first run eager,
-    # then hipGraph replays)
+    del dcs
     from mpsim_amd.program import CircuitProgram
     prog = CircuitProgram(circs, eng, svd)
     prog.run()
     prog.run()
     torch.cuda.synchronize()
     tg_list = []
-    for _ in range(args.steps):
+    for _ in range(max(steps, 3)):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         prog.run()
@@ -366,15 +373,27 @@ def run_uneven(args, eng, svd):
     torch.cuda.synchronize()
     tp = (time.perf_counter() - t0) / len(sample)
     value = len(circs) / tb
-    line = base_line("batched uneven circuits/s vs one-at-a-time GPU path", value, "circuits/s",
-                     args, 1e3 * tb, {"workload": f"{len(circs)} seeded circuits, n in [12,64], "
-                                      "depth in [4,24], chi cap in {32,64,128,256}",
-                                      "circuits": len(circs), "two_site_updates": updates})
-    line["extra"] = {"plain_gpu_circuits_per_s": round(1.0 / tp, 4),
-                     "plain_sample": f"first {len(sample)} circuits, one at a time",
-                     "batched_vs_plain": round(value * tp, 3),
-                     "updates_per_s": round(updates / tb, 2), **graph_info,
-                     "graph_vs_plain": round(len(circs) / tg * tp, 3)}
+    extra = {"workload": f"{len(circs)} seeded circuits, n in [12,64], depth in [4,24], "
+                         "chi cap in {32,64,128,256}",
+             "circuits": len(circs), "two_site_updates": updates,
+             "eager_circuits_per_s": round(value, 4),
+             "plain_gpu_circuits_per_s": round(1.0 / tp, 4),
+             "plain_sample": f"first {len(sample)} circuits, one at a time",
+             "batched_vs_plain": round(value * tp, 3),
+             "updates_per_s": round(updates / tb, 2), **graph_info,
+             "graph_vs_plain": round(len(circs) / tg * tp, 3)}
+    return tb, extra
+
+
+def run_uneven(args, eng, svd):
+    """Batched uneven circuits on their own line: value = eager circuits/s."""
+    tb, extra = uneven_leg(args, eng, svd, args.steps, args.warmup)
+    line = base_line("batched uneven circuits/s vs one-at-a-time GPU path", extra["eager_circuits_per_s"],
+                     "circuits/s", args, 1e3 * tb,
+                     {"workload": extra["workload"], "circuits": extra["circuits"],
+                      "two_site_updates": extra["two_site_updates"]})
+    line["extra"] = {k: v for k, v in extra.items()
+                     if k not in ("workload", "circuits", "two_site_updates", "eager_circuits_per_s")}
     emit(line)
 
 
@@ -681,6 +700,12 @@ def main():
     if not args.no_served:
         served = serve_leg(args, eng, svd)
 
+    uneven = None
+    if not args.no_uneven:
+        # the uneven-batch configuration beside the metric: eager ticks, one
+        # captured program, the plain path on a sample (2 eager passes, 3 replays)
+        _, uneven = uneven_leg(args, eng, svd, steps=2, warmup=1)
+
     cpu = None
     if not args.no_cpu:
         from oracle.cpu_baseline import blas_threads, cpu_model, time_plain16, time_updates
@@ -727,6 +752,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "served": served,
+        "uneven": uneven,
         "extra": extra,
     }
     print(json.dumps(line), flush=True)
