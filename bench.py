@@ -54,9 +54,10 @@ def parse():
     ap.add_argument("--fp32-sweeps", type=int, default=0,
                     help="Gram method: > 0 adds an FP32 phase of at most this many sweeps")
     ap.add_argument("--workload", default="large",
-                    choices=["large", "uneven", "served", "plain16", "qutrit", "copy", "accuracy",
-                             "hbm"],
-                    help="large (default, the metric line) | uneven | served | plain16 | qutrit")
+                    choices=["large", "uneven", "served", "qutrit", "copy", "hbm"],
+                    help="large (default, the metric line) | uneven | served | qutrit | copy (counter "
+                         "calibration) | hbm (memory-bound kernels); errors against FP64: "
+                         "tests/accuracy_report.py")
     ap.add_argument("--circuits", type=int, default=512, help="uneven: circuits in the batch")
     ap.add_argument("--plain-sample", type=int, default=16,
                     help="uneven/served: circuits timed one at a time on the plain path")
@@ -83,49 +84,6 @@ def base_line(metric, value, unit, args, ms_per_step, config, dtype=None):
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype or ("c64+f32" if getattr(args, "fp32", False) else "c64+f64"),
             "data": "synthetic (seeded Haar gates)", "config": config}
-
-
-def run_accuracy(args, eng, svd):
-    """Error against FP64 for every workload (the oracle is the checker):
-    plain 16-qubit circuit vs the FP64 state vector (max-abs); the 64-qubit
-    chi=256 circuit, sampled uneven circuits, sampled served requests and a
-    scaled qutrit circuit vs the FP64 restatement (gauge-invariant overlap
-    error 1 - |<ref|psi>|/(|ref||psi|), and relative norm error)."""
-    import numpy as np
-    from mpsim_amd.batch import prepare, run_batched
-    from mpsim_amd.circuits import (brickwork_cnot_u2, brickwork_haar, poisson_requests,
-                                    uneven_batch)
-    from oracle import statevector as sv
-    from oracle.mps_oracle import OracleMPS, gauge_invariant_error
-
-    def gpu(circ):
-        dc = prepare(circ, eng)
-        run_batched([dc], eng, svd)
-        return [t.cpu().numpy() for t in dc.mps.tensors], dc
-
-    def vs_oracle(circ):
-        t0 = time.perf_counter()
-        mine, _ = gpu(circ)
-        ora = OracleMPS(circ.n, circ.d, dtype_mode="c128")
-        ora.apply_ops(circ.ops, maxsvals=circ.maxsvals)
-        err, nr, nx = gauge_invariant_error(ora.tensors, mine)
-        return {"overlap_error": float(err), "norm_rel_error": abs(nr - nx) / nr,
-                "seconds": round(time.perf_counter() - t0, 2)}
-
-    out = {}
-    c1 = brickwork_cnot_u2(16, 8, seed=1)
-    t1, dc1 = gpu(c1)
-    psi = dc1.mps.wavefunction()
-    out["plain16_maxabs_vs_fp64_statevector"] = float(np.max(np.abs(psi - sv.run(c1.ops, 16))))
-    out["large64_chi256"] = vs_oracle(brickwork_haar(64, args.depth or 20, 2, maxsvals=256, seed=2))
-    out["uneven_sample"] = [dict(vs_oracle(c), name=c.name, n=c.n, cap=c.maxsvals)
-                            for c in uneven_batch(args.plain_sample)[:args.plain_sample]]
-    out["served_sample"] = [dict(vs_oracle(r.circuit), rid=r.rid, n=r.circuit.n,
-                                 cap=r.circuit.maxsvals)
-                            for r in [q for q in poisson_requests(limit=40)
-                                      if q.circuit.maxsvals <= 128][:3]]
-    out["qutrit_scaled_n12_d10_chi81"] = vs_oracle(brickwork_haar(12, 10, 3, maxsvals=81, seed=4))
-    emit({"workload": "accuracy", "results": out})
 
 
 def run_copy(args, eng):
@@ -580,8 +538,6 @@ def main():
         return run_copy(args, eng)
     if args.workload == "hbm":
         return run_hbm(args, eng, svd)
-    if args.workload == "accuracy":
-        return run_accuracy(args, eng, svd)
     if args.workload == "uneven":
         return run_uneven(args, eng, svd)
     if args.workload == "served":
