@@ -113,7 +113,7 @@ def ws_bound(C: int, R: int) -> int:
     bytes) and flags, and the mixed Gram method's second Cp x max(Cp, R)
     region (the FP32 Gram, then X1 = Y X)."""
     cp = C + 31
-    return 16 * cp * (R + cp + 4) + 32 * cp * max(cp, R) + 16 * cp * 64 + 2560
+    return 16 * cp * (R + cp + 4) + 32 * cp * (cp if cp > R else R) + 16 * cp * 64 + 2560
 
 
 def _ws_estimate(it: TwoSiteItem) -> int:
@@ -159,11 +159,12 @@ def tick(engine: Engine, dcs: Sequence[DeviceCircuit], svd: Optional[SvdConfig] 
         dc.t += 1
     for s in range(0, len(one), MAX_GROUP):
         engine.one_site_rows(one[s:s + MAX_GROUP])
+    est = [_ws_estimate(it) for it in two] if len(two) > 1 else [0] * len(two)
     start = 0
     while start < len(two):
         end, acc = start, 0
-        while end < len(two) and (end == start or acc + _ws_estimate(two[end]) <= ws_budget):
-            acc += _ws_estimate(two[end])
+        while end < len(two) and (end == start or acc + est[end] <= ws_budget):
+            acc += est[end]
             end += 1
         res = engine.two_site(two[start:end], svd)
         for (dc, i), it, r in zip(owners[start:end], two[start:end], res):

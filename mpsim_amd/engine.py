@@ -75,7 +75,7 @@ class SvdConfig:
                            self.unconverged or None, self.method, self.fp32_sweeps)
 
 
-@dataclass
+@dataclass(slots=True)
 class TwoSiteItem:
     a: "object"            # torch [chi_l, d, chi_m]
     b: "object"            # torch [chi_m, d, chi_r]
@@ -210,6 +210,7 @@ class Engine:
         # shapes and input pointers (one pass), then every output carved out of
         # one allocation and the descriptor table filled column by column
         cls_, ds, cms, crs, ks, pa, pb, pg, fl = ([] for _ in range(9))
+        gate_ptr: Dict[int, int] = {}  # a batch shares few gate tensors
         for it in items:
             if isinstance(it, SplitItem):
                 cl, cr = it.mat.shape
@@ -220,7 +221,10 @@ class Engine:
                 cm2, d2, cr = it.b.shape
                 if cm != cm2 or d != d2:
                     raise ValueError(f"site tensors do not share a bond: {tuple(it.a.shape)} {tuple(it.b.shape)}")
-                pa.append(nat.ptr(it.a)); pb.append(nat.ptr(it.b)); pg.append(nat.ptr(it.gate))
+                g = gate_ptr.get(id(it.gate))
+                if g is None:
+                    g = gate_ptr[id(it.gate)] = nat.ptr(it.gate)
+                pa.append(nat.ptr(it.a)); pb.append(nat.ptr(it.b)); pg.append(g)
             cls_.append(cl); ds.append(d); cms.append(cm); crs.append(cr); ks.append(it.k)
             fl.append(1 if it.absorb_left else 0)
         cl_a, d_a, cr_a, k_a = (np.asarray(x, dtype=np.int64) for x in (cls_, ds, crs, ks))
@@ -236,7 +240,7 @@ class Engine:
         disc = torch.empty(n, dtype=torch.float64, device=self.dev)
         bufs = [torch.empty(max(int(m), 1), dtype=torch.complex64, device=self.dev)
                 for m in length.tolist()]
-        ptr0 = np.array([nat.ptr(b) for b in bufs], dtype=np.int64)
+        ptr0 = np.array([b.data_ptr() for b in bufs], dtype=np.int64)  # fresh, contiguous
         tab = np.zeros(n, dtype=nat.TWO_SITE_OP)
         tab["a"], tab["b"], tab["gate"] = pa, pb, pg
         tab["a_out"] = np.where(na > 0, ptr0, 0)
