@@ -468,3 +468,53 @@ def test_singular_values_of_one_update_vs_lapack():
     assert rel <= 1e-4, rel
     disc = float(res.discarded.cpu())
     assert abs(disc - np.sum(s_ref[256:] ** 2)) <= 1e-4 * np.sum(s_ref ** 2)
+
+
+def test_two_site_shape_sweep_vs_lapack():
+    """One grouped library call over 40 seeded two-site problems of mixed
+    qudit dimension (d in {2,3,4,5,7}: specialised and generic theta kernels
+    in the same call), bond dimensions 1..64 (including chain-end 1s), every
+    kind of k (0, 1, a cut, all values) and both absorb directions, each
+    against numpy's FP64 SVD of the same theta (formed from the complex64
+    inputs): kept singular values within 1e-4 relative + 1e-6 ||theta||_F,
+    discarded weight, the truncated product A'B' equal to the best rank-k
+    approximation within 1e-4 ||theta||_F, and the unscaled factor
+    orthonormal (core.py:1132-1145)."""
+    m = _mpsim()
+    import torch
+    from mpsim_amd.engine import Engine, TwoSiteItem
+    rng = np.random.RandomState(2024)
+    eng = Engine.get()
+    bonds = [1, 2, 3, 7, 16, 33, 64]
+    cases, items = [], []
+    for _ in range(40):
+        d = int(rng.choice([2, 3, 4, 5, 7]))
+        cl, cm, cr = (int(rng.choice(bonds)) for _ in range(3))
+        M, N = d * cl, d * cr
+        q = min(M, N)
+        k = int(rng.choice([0, 1, q, int(rng.randint(1, q + 1))]))
+        left = bool(rng.randint(2))
+        A = ((rng.randn(cl, d, cm) + 1j * rng.randn(cl, d, cm)) / np.sqrt(2 * d * cm)).astype(np.complex64)
+        B = ((rng.randn(cm, d, cr) + 1j * rng.randn(cm, d, cr)) / np.sqrt(2)).astype(np.complex64)
+        G = m.gates.haar_unitary(d * d, rng)
+        cases.append((d, cl, cm, cr, k, left, A, B, G))
+        items.append(TwoSiteItem(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(),
+                                 eng.gate(G), k, absorb_left=left, want_s=True))
+    res = eng.two_site(items)
+    torch.cuda.synchronize()
+    for (d, cl, cm, cr, k, left, A, B, G), r in zip(cases, res):
+        tag = (d, cl, cm, cr, k, left)
+        theta = np.einsum("abpq,lpqr->labr", G.reshape(d, d, d, d).astype(np.complex128),
+                          np.tensordot(A.astype(np.complex128), B.astype(np.complex128),
+                                       axes=([2], [0]))).reshape(d * cl, d * cr)
+        u, s_ref, vh = np.linalg.svd(theta, full_matrices=False)
+        fro = np.linalg.norm(theta)
+        a = r.a_out.cpu().numpy().reshape(d * cl, k)
+        b = r.b_out.cpu().numpy().reshape(k, d * cr)
+        s = r.s.cpu().numpy().astype(np.float64) if k else np.zeros(0)
+        assert np.all(np.abs(s - s_ref[:k]) <= 1e-4 * s_ref[:k] + 1e-6 * fro), tag
+        assert abs(float(r.discarded.cpu()) - np.sum(s_ref[k:] ** 2)) <= 1e-5 * fro ** 2, tag
+        best = (u[:, :k] * s_ref[:k]) @ vh[:k]
+        assert np.linalg.norm(a.astype(np.complex128) @ b - best) <= 1e-4 * max(fro, 1e-30), tag
+        unscaled = b @ b.conj().T if left else a.conj().T @ a
+        assert np.allclose(unscaled, np.eye(k), atol=1e-5), tag
