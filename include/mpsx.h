@@ -119,7 +119,9 @@ typedef struct {
     int32_t zbytes;    /* bytes per working-matrix element: 16 (complex
                           FP64 rotations) or 8 (complex FP32)             */
     int32_t method;    /* 0: one-sided Jacobi on the rows of [X | Y];
-                          1: two-sided Jacobi on the Gram H = X X^H       */
+                          1: two-sided Jacobi on the Gram H = X X^H;
+                          2: the same with an FP32 first phase;
+                          3: method 1 after a Householder QR of X         */
 } mpsx_plan;
 
 typedef struct {
@@ -168,6 +170,22 @@ typedef struct {
                            (max_sweeps bounds those). 0 (default) or < 0:
                            FP64 rounds only. The results agree with the FP64
                            rounds' to FP64 rounding of the converged rows. */
+    int32_t precondition;/* Householder QR before the FP64 Gram method (no
+                           counterpart in the reference, whose SVD is
+                           LAPACK's): X P = Q R with the columns ordered by
+                           decreasing norm, then the same Jacobi on
+                           R R^H = Q^H X X^H Q starting from Y = Q^H. Fewer
+                           sweeps where the spectrum of theta decays; more
+                           where it is flat.
+                           0 (default): none;
+                           1: two-site problems whose middle bond is at least
+                           the smaller outer bond (chi_m >= min(chi_l,
+                           chi_r): theta built from an already truncated
+                           bond);
+                           2: every Gram problem (plain splits included).
+                           Only problems of at most 512 padded rows and
+                           4096 columns; the others keep the plain Gram
+                           method (plan.method 3 marks the preconditioned) */
 } mpsx_svd_opts;
 
 #define MPSX_SVD_AUTO 0

@@ -47,7 +47,8 @@ class SvdOpts(ctypes.Structure):
                 ("check_every", ctypes.c_int32), ("inner_sweeps", ctypes.c_int32),
                 ("tol", ctypes.c_double), ("block_switch", ctypes.c_int32),
                 ("rotation_fp32", ctypes.c_int32), ("unconverged", ctypes.c_void_p),
-                ("method", ctypes.c_int32), ("fp32_sweeps", ctypes.c_int32)]
+                ("method", ctypes.c_int32), ("fp32_sweeps", ctypes.c_int32),
+                ("precondition", ctypes.c_int32)]
 
 
 class MpsxError(RuntimeError):

@@ -15,7 +15,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmpsx.so")
-SOURCES = ["api.hip", "noalloc.hip", "site.hip", "gemm.hip", "twosite.hip"]
+SOURCES = ["api.hip", "noalloc.hip", "site.hip", "gemm.hip", "twosite.hip", "qr.hip"]
 ARCH = os.environ.get("MPSX_ARCH", "gfx950")
 
 

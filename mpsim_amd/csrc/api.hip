@@ -67,11 +67,12 @@ struct Resolved {
     int zbytes;
     int method;
     int fp32_sweeps;  // sweep limit of the Gram method's FP32 phase; 0: no such phase
+    int precondition;  // 0 none, 1 two-site problems of saturated bonds, 2 every Gram problem
 };
 
 Resolved resolve(const mpsx_svd_opts *o) {
     Resolved r{0, kDefaultMaxSweeps, 1, kDefaultInnerSweeps, 0.0, 64, nullptr, false, 0.0, 16,
-               MPSX_SVD_AUTO, 0};
+               MPSX_SVD_AUTO, 0, 0};
     if (o) {
         if (o->block) r.block = o->block;
         if (o->max_sweeps) r.max_sweeps = o->max_sweeps;
@@ -83,6 +84,7 @@ Resolved resolve(const mpsx_svd_opts *o) {
         r.fp32 = o->rotation_fp32 != 0;
         r.method = o->method;
         r.fp32_sweeps = o->fp32_sweeps > 0 ? o->fp32_sweeps : 0;
+        r.precondition = o->precondition;
     }
     if (r.tol <= 0) r.tol = r.fp32 ? kDefaultTol32 : kDefaultTol64;
     r.null_rel = r.fp32 ? kNullRel32 : kNullRel64;
@@ -338,6 +340,10 @@ int plan_shape(const mpsx_two_site_op &o, int i, const Resolved &r, mpsx_plan *p
     p->LD = (int32_t)align_up((int64_t)p->R + p->Cp, 4);
     p->zbytes = r.zbytes;
     p->method = p->nb > 0 ? plan_method(r, p->b, p->nb) : 0;
+    if (p->method == 1 && p->Cp <= QR_MAX_CP && p->R <= QR_MAX_R &&
+        (r.precondition == 2 ||
+         (r.precondition == 1 && !split && o.chi_m >= std::min(o.chi_l, o.chi_r))))
+        p->method = 3;
     return MPSX_OK;
 }
 
@@ -348,6 +354,8 @@ int check_options(const Resolved &r) {
     if (r.method != MPSX_SVD_AUTO && r.method != MPSX_SVD_ROWS)
         return fail(MPSX_EINVAL, "method %d not in {0 (auto), 1 (rows)}", r.method);
     if (r.fp32_sweeps > 64) return fail(MPSX_EINVAL, "fp32_sweeps above 64");
+    if (r.precondition < 0 || r.precondition > 2)
+        return fail(MPSX_EINVAL, "precondition %d not in {0, 1, 2}", r.precondition);
     return MPSX_OK;
 }
 
@@ -412,6 +420,7 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
     bool has_d[THETA_GEN_DMAX + 1] = {};  // index 1: plain splits
     bool has_b[17] = {false};  // one-sided problems by block width
     bool has_gram = false, has_mixed = false;
+    int qr_c = 0, qr_rc = 0, qr_np = 0;  // QR-preconditioned problems: max C, R + C, panels
     for (int i = 0; i < count; ++i) {
         const mpsx_two_site_op &o = h[i];
         const mpsx_plan &p = o.plan;
@@ -443,6 +452,11 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
             gram_y = std::max(gram_y, p.nb / 4 + (p.Cp + 63) / 64);
             gram_c = std::max(gram_c, (p.R + 31) / 32);
         }
+        if (p.nb > 0 && p.method == 3) {
+            qr_c = std::max(qr_c, (int)p.C);
+            qr_rc = std::max(qr_rc, (int)(p.R + p.C));
+            qr_np = std::max(qr_np, qr_panels(p));
+        }
     }
     if (tiles > kGridLimit || pairs > kGridLimit || rows > kGridLimit || outs > kGridLimit)
         return fail(MPSX_EINVAL, "batch too large for one launch");
@@ -467,6 +481,19 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
     if (rows > 0) {
         HIP_TRY(timed("k_rownorm", stream, [&] { return launch_rownorm(stream, f32, d, count, ws, rows); }), "k_rownorm");
         HIP_TRY(timed("k_fro", stream, [&] { return launch_fro(stream, d, count, ws, r.null_rel); }), "k_fro");
+    }
+    if (qr_c > 0) {
+        // [X P | I] -> [R | Q^H] for the preconditioned problems, X0 saved
+        HIP_TRY(timed("k_qr_order", stream, [&] { return launch_qr_order(stream, d, count, ws); }), "k_qr_order");
+        HIP_TRY(timed("k_qr_permute", stream, [&] { return launch_qr_permute(stream, d, count, ws, rows); }),
+                "k_qr_permute");
+        for (int panel = -1; panel < qr_np; ++panel) {
+            const int strips = panel < 0 ? 1 : (qr_rc - 16 * (panel + 1) + 15) / 16;
+            if (strips > 0)
+                HIP_TRY(timed("k_qr_step", stream,
+                              [&] { return launch_qr_step(stream, d, count, ws, panel, strips, qr_c); }),
+                        "k_qr_step");
+        }
     }
 
     int sweeps = 0;

@@ -38,6 +38,14 @@ MPSX_DEV double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
 MPSX_DEV double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 MPSX_DEV double2 zscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 
+// FP64 matrix-core product D = A B + C of 16x16x4 tiles. Lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16]; register r of lane l holds
+// D[4 r + l / 16][l % 16].
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+MPSX_DEV mfma_d4 mfma16d(double a, double b, mfma_d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
 // Round-robin ("circle method") pairing of n (even) items: in round r
 // (0 <= r < n-1) slot p (0 <= p < n/2) pairs item lo with item hi; every
 // item appears once per round and every unordered pair meets exactly once
@@ -132,9 +140,27 @@ MPSX_HD int64_t qflag_off(const mpsx_plan &p) { return qmat_off(p) + 2 * 16LL * 
 MPSX_HD int64_t t_off(const mpsx_plan &p) {
     return align_up(qflag_off(p) + 2 * 4LL * (p.nb / 2), 256);
 }
+// Problems of the QR-preconditioned Gram method (plan.method 3) add, at the
+// same offset as region T:
+//   X0[Cp][R] (double2, the unpermuted X, read by the final X <- Y X0;
+//   padded rows zero) | V[2][Cp][16] (double2, the Householder vectors of
+//   a 16-column panel, for even and odd panels) | T[2][16][16] (double2,
+//   the panel's triangular factor) | order[R] (int32, columns by
+//   decreasing norm)
+constexpr int QR_CHUNKS = 4;     // 16-row chunks a wave of the QR step holds
+constexpr int QR_MAX_CP = 128 * QR_CHUNKS;  // rows held by one workgroup's registers
+constexpr int QR_MAX_R = 4096;   // a row staged in LDS
+MPSX_HD int64_t qx0_off(const mpsx_plan &p) { return t_off(p); }
+MPSX_HD int64_t qv_off(const mpsx_plan &p) { return qx0_off(p) + align_up(16LL * p.Cp * p.R, 256); }
+MPSX_HD int64_t qt_off(const mpsx_plan &p) { return qv_off(p) + 2 * 16LL * 16 * p.Cp; }
+MPSX_HD int64_t qorder_off(const mpsx_plan &p) { return qt_off(p) + 2 * 16LL * 256; }
+// Householder reflectors of the QR: one per column of X up to the last row.
+MPSX_HD int qr_reflectors(const mpsx_plan &p) { return p.C - 1 < p.R ? p.C - 1 : p.R; }
+MPSX_HD int qr_panels(const mpsx_plan &p) { return (qr_reflectors(p) + 15) / 16; }
 MPSX_HD int64_t problem_bytes(const mpsx_plan &p) {
     if (p.method == 0) return align_up(z_off(p) - p.ws_off + (int64_t)p.zbytes * p.Cp * p.LD, 256);
     if (p.method == 1) return align_up(qflag_off(p) - p.ws_off + 2 * 4LL * (p.nb / 2), 256);
+    if (p.method == 3) return align_up(qorder_off(p) - p.ws_off + 4LL * p.R, 256);
     return align_up(t_off(p) - p.ws_off + 16 * h_elems(p), 256);
 }
 

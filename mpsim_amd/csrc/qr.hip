@@ -1,0 +1,392 @@
+// QR preconditioning of the Gram-method Jacobi (plan.method 3), gfx950.
+//
+// The Gram method orthogonalises the C rows of X (C x R, FP64 complex) by
+// two-sided Jacobi on H = X X^H, accumulating the rotations in Y (starting
+// as the identity) and finishing with X <- Y X. Preconditioned, it first
+// takes a Householder QR of X with its columns in a fixed order P (by
+// decreasing norm), X P = Q R, applied to the whole working row [X P | Y]:
+//   [X P | I]  ->  [R | Q^H]
+// so that the Jacobi starts from H = R R^H = Q^H (X X^H) Q and Y = Q^H, and
+// the rotations W it finds give Y = W Q^H. The end is unchanged except that
+// X <- Y X0 reads the saved, unpermuted X0: W Q^H X0 = W R P^T has
+// orthogonal rows, the absorbed factor, and U = Y^H = Q W^H is exactly
+// unitary (products of unitary reflectors and rotations), so nothing is
+// divided by a singular value. P only decides which Q the QR finds; it is
+// never undone. R R^H is nearer diagonal than X X^H when the spectrum
+// decays (the Drmac-Veselic preconditioner of one-sided Jacobi), which
+// saves sweeps.
+//
+// Householder reflector of x (column below and at the pivot row q):
+//   H = I - tau v v^H,  v_q = 1,  H x = beta e_q,
+//   beta = -e^{i arg x_q} ||x||,  tau = 1 + |x_q| / ||x|| (real, H Hermitian
+//   and unitary),  v = x / (x_q - beta) below q;  tau = 0 when x is zero
+//   below q.
+// Sixteen reflectors form a panel, Q_p = H_0 ... H_15 = I - V T V^H with T
+// upper triangular (T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] V^H v_j),
+// and the columns right of the panel take Q_p^H A = A - V (T^H (V^H A)).
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "common.h"
+#include "launch.h"
+
+#define OP_FIELD(f) offsetof(mpsx_two_site_op, f)
+
+namespace {
+
+MPSX_DEV double2 *zq_ptr(char *ws, const mpsx_plan &p) {
+    return reinterpret_cast<double2 *>(ws + z_off(p));
+}
+MPSX_DEV double2 *qv_ptr(char *ws, const mpsx_plan &p, int panel) {
+    return reinterpret_cast<double2 *>(ws + qv_off(p)) + (int64_t)(panel & 1) * 16 * p.Cp;
+}
+MPSX_DEV double2 *qt_ptr(char *ws, const mpsx_plan &p, int panel) {
+    return reinterpret_cast<double2 *>(ws + qt_off(p)) + (panel & 1) * 256;
+}
+MPSX_DEV int *qorder_ptr(char *ws, const mpsx_plan &p) {
+    return reinterpret_cast<int *>(ws + qorder_off(p));
+}
+// Component-wise select by value (a conditional on two double2 lvalues
+// selects an address and keeps the register arrays in memory).
+MPSX_DEV double2 zsel(bool c, double2 x, double2 y) {
+    return make_double2(c ? x.x : y.x, c ? x.y : y.y);
+}
+MPSX_DEV double2 shfl_xor_z(double2 v, int m) {
+    return make_double2(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+
+}  // namespace
+
+// Column order of X, one workgroup per problem: squared column norms in
+// FP64, then the rank of each column under (norm descending, index
+// ascending); order[rank] = column. A NaN norm ranks last, so the order is
+// a permutation whatever the data.
+__global__ void __launch_bounds__(1024) k_qr_order(const mpsx_two_site_op *ops, char *ws) {
+    __shared__ double nrm[QR_MAX_R];
+    __shared__ double part[4][256];
+    const mpsx_plan p = ops[blockIdx.x].plan;
+    if (p.method != 3) return;
+    const int C = p.C, R = p.R, LD = p.LD;
+    const double2 *Z = zq_ptr(ws, p);
+    const int tid = threadIdx.x, g = tid >> 8, t = tid & 255;
+    for (int c0 = 0; c0 < R; c0 += 256) {
+        const int c = c0 + t;
+        double s = 0.0;
+        if (c < R) {
+#pragma unroll 8
+            for (int r = g; r < C; r += 4) {
+                const double2 v = Z[(int64_t)r * LD + c];
+                s = fma(v.x, v.x, fma(v.y, v.y, s));
+            }
+        }
+        part[g][t] = s;
+        __syncthreads();
+        if (g == 0 && c < R) {
+            const double v = (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
+            nrm[c] = v == v ? v : -1.0;
+        }
+        __syncthreads();
+    }
+    int *order = qorder_ptr(ws, p);
+    for (int c = tid; c < R; c += 1024) {
+        const double v = nrm[c];
+        int rank = 0;
+        for (int o = 0; o < R; ++o) {
+            const double w = nrm[o];
+            rank += (w > v) || (w == v && o < c);
+        }
+        order[rank] = c;
+    }
+}
+
+// One workgroup per padded row: X0[row] <- X[row] (zero for padded rows),
+// then X[row][j] <- X0[row][order[j]].
+__global__ void __launch_bounds__(256) k_qr_permute(const mpsx_two_site_op *ops, int count, char *ws) {
+    __shared__ double2 xs[QR_MAX_R];
+    const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
+                               OP_FIELD(plan.row_base), blockIdx.x);
+    const mpsx_plan p = ops[pi].plan;
+    if (p.method != 3) return;
+    const int row = (int)(blockIdx.x - p.row_base);
+    if (row >= p.Cp) return;
+    const int R = p.R;
+    double2 *x0 = reinterpret_cast<double2 *>(ws + qx0_off(p)) + (int64_t)row * R;
+    if (row >= p.C) {
+        for (int c = threadIdx.x; c < R; c += 256) x0[c] = make_double2(0.0, 0.0);
+        return;
+    }
+    double2 *z = zq_ptr(ws, p) + (int64_t)row * p.LD;
+    for (int c = threadIdx.x; c < R; c += 256) {
+        const double2 v = z[c];
+        xs[c] = v;
+        x0[c] = v;
+    }
+    __syncthreads();
+    const int *order = qorder_ptr(ws, p);
+    for (int j = threadIdx.x; j < R; j += 256) {
+        int src = order[j];
+        if ((unsigned)src >= (unsigned)R) src = j;
+        z[j] = xs[src];
+    }
+}
+
+// One step of the blocked QR. x: problem; y: 16-column strip. Launch
+// `panel` >= 0 applies panel `panel`'s block reflector (V, T from buffer
+// panel & 1) to the rows panel*16 .. C-1 of the strip's columns
+// 16 (panel + 1 + y) .. +15 (the X columns right of the panel, then the Y
+// columns; every column of [X | Y] right of the panel up to R + C - 1);
+// strip 0 (the next panel's columns) then factors panel + 1 in registers
+// and writes its V and T to the other buffer. Launch -1 only factors panel
+// 0. Eight waves; wave w holds the 16-row chunks w, w + 8, ... of the
+// strip, chunk i as the four registers a[i][r] = (row 16 (w + 8 i) + 4 r +
+// lq, column l16) of lane (l16, lq), the matrix-core output layout.
+template <int CH>
+__global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, char *ws, int panel) {
+    __shared__ double2 Wp[8][16][17];  // per-wave partial V^H A
+    __shared__ double2 Wt[16][17];     // V^H A
+    __shared__ double2 Ws[16][17];     // T^H V^H A
+    __shared__ double2 Ts[16][17];     // T of the panel applied / factored
+    __shared__ double2 Ys[16][17];     // Ys[j][k] = v_k^H v_j (k < j), for T
+    __shared__ double taus[16];
+    __shared__ double red[8];
+    __shared__ double2 redz[8][16];
+    __shared__ double2 alpha_s;
+    __shared__ double2 vcol[8][16 * CH];  // per wave: v of the reflector, the wave's rows
+
+    const mpsx_plan p = ops[blockIdx.x].plan;
+    if (p.method != 3) return;
+    const int C = p.C, LD = p.LD, ncol = p.R + p.C;
+    const int np = qr_panels(p);
+    if (panel >= np) return;
+    const int c0 = 16 * (panel + 1 + (int)blockIdx.y);
+    if (c0 >= ncol) return;
+    const bool update = panel >= 0;
+    const bool factor = blockIdx.y == 0 && panel + 1 < np;
+    if (!update && !factor) return;
+    const int rb = update ? 16 * panel : 0;  // first row held
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, l16 = lane & 15, lq = lane >> 4;
+    const int col = c0 + l16;
+    const bool col_ok = col < ncol;
+    double2 *Z = zq_ptr(ws, p);
+
+    double2 a[CH][4];
+#pragma unroll
+    for (int i = 0; i < CH; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
+            a[i][r] = (row < C && col_ok) ? Z[(int64_t)row * LD + col] : make_double2(0.0, 0.0);
+        }
+
+    if (update) {
+        const double2 *V = qv_ptr(ws, p, panel);
+        const double2 *T = qt_ptr(ws, p, panel);
+        // W = V^H A over this wave's rows: A operand conj(V[row][refl]),
+        // B operand the strip; three real products per complex product
+        mfma_d4 m1 = {0.0, 0.0, 0.0, 0.0}, m2 = m1, m3 = m1;
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            const int base = rb + 16 * (w + 8 * i);
+            if (base >= C) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = base + 4 * r + lq;
+                const double2 v = row < C ? V[(int64_t)row * 16 + l16] : make_double2(0.0, 0.0);
+                m1 = mfma16d(v.x, a[i][r].x, m1);
+                m2 = mfma16d(v.y, a[i][r].y, m2);
+                m3 = mfma16d(v.x - v.y, a[i][r].x + a[i][r].y, m3);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            Wp[w][4 * r + lq][l16] = make_double2(m1[r] + m2[r], m3[r] - m1[r] + m2[r]);
+        if (tid < 256) Ts[tid >> 4][tid & 15] = T[tid];
+        __syncthreads();
+        if (tid < 256) {
+            const int i = tid >> 4, c = tid & 15;
+            double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) s = zadd(s, Wp[v][i][c]);
+            Wt[i][c] = s;
+        }
+        __syncthreads();
+        if (tid < 256) {  // Ws = T^H Wt
+            const int i = tid >> 4, c = tid & 15;
+            double2 s = make_double2(0.0, 0.0);
+            for (int k = 0; k <= i; ++k) {
+                const double2 t = Ts[k][i], x = Wt[k][c];
+                s.x += t.x * x.x + t.y * x.y;
+                s.y += t.x * x.y - t.y * x.x;
+            }
+            Ws[i][c] = s;
+        }
+        __syncthreads();
+        // A -= V Ws: A operand V[row][refl], B operand Ws[refl][column]
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            const int base = rb + 16 * (w + 8 * i);
+            if (base >= C) continue;
+            mfma_d4 n1 = {0.0, 0.0, 0.0, 0.0}, n2 = n1, n3 = n1;
+            const int vrow = base + l16;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const double2 v = vrow < C ? V[(int64_t)vrow * 16 + 4 * t + lq] : make_double2(0.0, 0.0);
+                const double2 x = Ws[4 * t + lq][l16];
+                n1 = mfma16d(v.x, x.x, n1);
+                n2 = mfma16d(v.y, x.y, n2);
+                n3 = mfma16d(v.x + v.y, x.x + x.y, n3);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                a[i][r].x -= n1[r] - n2[r];
+                a[i][r].y -= n3[r] - n1[r] - n2[r];
+            }
+        }
+    }
+
+    int ncols = 0, rf = 0;
+    if (factor) {
+        rf = 16 * (panel + 1);
+        ncols = min(16, qr_reflectors(p) - rf);
+        for (int j = 0; j < ncols; ++j) {
+            const int q = rf + j;
+            // squared norm of column j below row q (every lane for its own
+            // column; lane j's is used), and x_q
+            double s = 0.0;
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
+                    const double e = fma(a[i][r].x, a[i][r].x, a[i][r].y * a[i][r].y);
+                    s += row > q ? e : 0.0;
+                    if (row == q && l16 == j) alpha_s = a[i][r];
+                }
+            s += __shfl_xor(s, 16, 64);
+            s += __shfl_xor(s, 32, 64);
+            if (lq == 0 && l16 == j) red[w] = s;
+            __syncthreads();
+            const double s2 = ((red[0] + red[1]) + (red[2] + red[3])) + ((red[4] + red[5]) + (red[6] + red[7]));
+            const double2 al = alpha_s;
+            double tau = 0.0;
+            double2 beta = al, sc = make_double2(0.0, 0.0);
+            if (s2 > 0.0) {  // two long operations deep: the square roots, then the reciprocals
+                const double a2 = al.x * al.x + al.y * al.y;
+                const double aa = sqrt(a2), xn = sqrt(a2 + s2);
+                const double ia = 1.0 / aa, ix = 1.0 / xn, inv = 1.0 / (aa + xn);
+                const double2 u = aa > 0.0 ? make_double2(al.x * ia, al.y * ia) : make_double2(1.0, 0.0);
+                beta = make_double2(-u.x * xn, -u.y * xn);
+                tau = 1.0 + aa * ix;
+                sc = make_double2(u.x * inv, -u.y * inv);  // conj(u) / (|x_q| + ||x||)
+            }
+            // column j: beta at q, v below (kept in place of the zeros). Lane
+            // j of each row also puts v (1 at q, 0 above) in the wave's LDS
+            // column, from which every lane of the row reads it: a wave owns
+            // its rows, so no barrier is needed (LDS operations of one wave
+            // complete in order).
+            const bool cj = l16 == j;
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
+                    const double2 sv = zmul(a[i][r], sc);
+                    a[i][r] = zsel(cj, zsel(row == q, beta, zsel(row > q, sv, a[i][r])), a[i][r]);
+                    if (cj) vcol[w][16 * i + 4 * r + lq] = zsel(row > q, sv, make_double2(row == q ? 1.0 : 0.0, 0.0));
+                }
+            __builtin_amdgcn_wave_barrier();
+            double2 vv[CH][4];
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) vv[i][r] = vcol[w][16 * i + 4 * r + lq];
+            // g = v^H a for every column of the strip; columns k > j take
+            // a -= tau v g, columns k < j hold v_k and give conj(v_k^H v_j)
+            double2 g = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    g.x += vv[i][r].x * a[i][r].x + vv[i][r].y * a[i][r].y;
+                    g.y += vv[i][r].x * a[i][r].y - vv[i][r].y * a[i][r].x;
+                }
+            g = zadd(g, shfl_xor_z(g, 16));
+            g = zadd(g, shfl_xor_z(g, 32));
+            if (lq == 0) redz[w][l16] = g;
+            __syncthreads();
+            g = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) g = zadd(g, redz[v][l16]);
+            if (tid < 16 && tid < j) Ys[j][tid] = make_double2(g.x, -g.y);
+            if (tid == 0) taus[j] = tau;
+            const double2 tg = l16 > j ? zscale(g, tau) : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    a[i][r].x -= vv[i][r].x * tg.x - vv[i][r].y * tg.y;
+                    a[i][r].y -= vv[i][r].x * tg.y + vv[i][r].y * tg.x;
+                }
+        }
+        __syncthreads();
+        // T row k, one thread each (a compact loop: straight-line code run
+        // once per launch costs more in instruction fetch than it saves):
+        // T[k][k] = tau_k, T[k][j] = -tau_j sum_{m=k}^{j-1} T[k][m] Ys[j][m]
+        // for k < j < ncols, zero elsewhere
+        if (tid < 16) {
+            const int k = tid;
+            for (int j = 0; j < 16; ++j) Ts[k][j] = make_double2(j == k && k < ncols ? taus[k] : 0.0, 0.0);
+            for (int j = k + 1; j < ncols; ++j) {
+                double2 sum = make_double2(0.0, 0.0);
+                for (int m = k; m < j; ++m) sum = zadd(sum, zmul(Ts[k][m], Ys[j][m]));
+                Ts[k][j] = zscale(sum, -taus[j]);
+            }
+            double2 *T = qt_ptr(ws, p, panel + 1) + k * 16;
+            for (int j = 0; j < 16; ++j) T[j] = Ts[k][j];
+        }
+        // V rows rf .. C-1 of the panel
+        double2 *V = qv_ptr(ws, p, panel + 1);
+#pragma unroll
+        for (int i = 0; i < CH; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
+                if (row < rf || row >= C) continue;
+                const int q = rf + l16;
+                double2 v = make_double2(0.0, 0.0);
+                if (l16 < ncols) v = row > q ? a[i][r] : make_double2(row == q ? 1.0 : 0.0, 0.0);
+                V[(int64_t)row * 16 + l16] = v;
+            }
+    }
+
+    if (!col_ok) return;
+#pragma unroll
+    for (int i = 0; i < CH; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
+            if (row >= C) continue;
+            const bool below = factor && l16 < ncols && row > rf + l16;
+            Z[(int64_t)row * LD + col] = below ? make_double2(0.0, 0.0) : a[i][r];
+        }
+}
+
+hipError_t launch_qr_order(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws) {
+    k_qr_order<<<count, 1024, 0, s>>>(d, ws);
+    return hipGetLastError();
+}
+
+hipError_t launch_qr_permute(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
+                             int64_t rows) {
+    k_qr_permute<<<(unsigned)rows, 256, 0, s>>>(d, count, ws);
+    return hipGetLastError();
+}
+
+hipError_t launch_qr_step(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int panel,
+                          int strips, int max_c) {
+    if (max_c > 128 * QR_CHUNKS) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)count, (unsigned)strips);
+    k_qr_step<QR_CHUNKS><<<grid, 512, 0, s>>>(d, ws, panel);
+    return hipGetLastError();
+}

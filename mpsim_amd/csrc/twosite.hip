@@ -720,11 +720,6 @@ k_jacobi_round32(const mpsx_two_site_op *ops, const int *owner, char *ws, int ro
 // three operand arrays would halve the occupancy, which measured slower).
 // Each wave accumulates a partial Gram over its columns; the four partials
 // are summed in a fixed order, (w2 + w0) + (w3 + w1), through LDS.
-typedef double mfma_d4 __attribute__((ext_vector_type(4)));
-
-MPSX_DEV mfma_d4 mfma16d(double a, double b, mfma_d4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
 
 template <int B2>
 __global__ void __launch_bounds__(256)
@@ -980,6 +975,12 @@ MPSX_DEV float2 *h32_ptr(char *ws, const mpsx_plan &p) {
 }
 MPSX_DEV double2 *t64_ptr(char *ws, const mpsx_plan &p) {
     return reinterpret_cast<double2 *>(ws + t_off(p));
+}
+// The copy of X (Cp x R, row stride R) that the end's X <- Y X reads: made
+// in the H region after the sweeps, or, QR-preconditioned, the X0 saved
+// before the QR.
+MPSX_DEV const double2 *xc_ptr(char *ws, const mpsx_plan &p) {
+    return p.method == 3 ? reinterpret_cast<const double2 *>(ws + qx0_off(p)) : h_ptr(ws, p);
 }
 // Problems the Gram round kernels work on: in the FP32 phase (F32H) the
 // mixed ones (method 2), in the FP64 rounds every Gram problem (methods 1
@@ -1605,7 +1606,7 @@ k_hxcopy(const mpsx_two_site_op *ops, int count, char *ws) {
     const int pi = find_record(reinterpret_cast<const char *>(ops), count, sizeof(mpsx_two_site_op),
                                OP_FIELD(plan.row_base), blockIdx.x);
     const mpsx_plan p = ops[pi].plan;
-    if (p.method < 1) return;
+    if (p.method < 1 || p.method == 3) return;  // method 3 kept X in X0
     const int row = (int)(blockIdx.x - p.row_base);
     if (row >= p.Cp) return;
     const double2 *z = z_ptr<double>(ws, p) + (int64_t)row * p.LD;
@@ -1635,7 +1636,7 @@ k_hxgemm(const mpsx_two_site_op *ops, const int *owner, char *ws) {
     const int col = col0 + 16 * tn + l16;
     const bool ok = col < R;
     double2 *Z = z_ptr<double>(ws, p);
-    const double2 *Xc = MODE == 0 ? h_ptr(ws, p) : Z;  // B = X[k][col]
+    const double2 *Xc = MODE == 0 ? xc_ptr(ws, p) : Z;  // B = X[k][col]
     const int ldx = MODE == 0 ? R : LD;
     double2 *Out = MODE == 0 ? Z : t64_ptr(ws, p);
     const int ldo = MODE == 0 ? LD : R;
@@ -1703,7 +1704,7 @@ k_hgemm(const mpsx_two_site_op *ops, const int *owner, char *ws) {
     // that form B = X^H; for MODES 2, 3 B = X[k][j]
     const double2 *Asrc = gram ? (MODE == 0 ? Z : t64_ptr(ws, p)) : Z + R;
     const int lda = gram ? (MODE == 0 ? LD : R) : LD;
-    const double2 *Bsrc = gram ? Asrc : (MODE == 2 ? h_ptr(ws, p) : Z);
+    const double2 *Bsrc = gram ? Asrc : (MODE == 2 ? xc_ptr(ws, p) : Z);
     const int ldb = gram ? lda : (MODE == 2 ? R : LD);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, lq = lane >> 4;

@@ -68,11 +68,13 @@ class SvdConfig:
     fp32: bool = False      # FP32 working matrix (faster, ~1e-6 relative error per update)
     method: int = 0         # 0: Gram method for FP64 32-row pivots; 1: one-sided everywhere
     fp32_sweeps: int = 0    # Gram method: > 0 adds an FP32 phase of at most this many sweeps
+    precondition: int = 0   # Householder QR before the Gram method: 0 none, 1 saturated bonds, 2 all
 
     def to_c(self) -> nat.SvdOpts:
         return nat.SvdOpts(self.block, self.max_sweeps, self.check_every, self.inner_sweeps,
                            self.tol, self.block_switch, 1 if self.fp32 else 0,
-                           self.unconverged or None, self.method, self.fp32_sweeps)
+                           self.unconverged or None, self.method, self.fp32_sweeps,
+                           self.precondition)
 
 
 @dataclass(slots=True)

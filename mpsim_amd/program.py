@@ -190,7 +190,8 @@ class CircuitProgram:
         return SvdConfig(block=c.block, max_sweeps=sweeps if sweeps else c.max_sweeps,
                          check_every=0 if sweeps else 1, inner_sweeps=c.inner_sweeps, tol=c.tol,
                          block_switch=c.block_switch, unconverged=int(self.unconverged.data_ptr()),
-                         fp32=c.fp32, method=c.method, fp32_sweeps=c.fp32_sweeps).to_c()
+                         fp32=c.fp32, method=c.method, fp32_sweeps=c.fp32_sweeps,
+                         precondition=c.precondition).to_c()
 
     def _plan(self, tab: np.ndarray) -> int:
         nbytes = ctypes.c_size_t(0)

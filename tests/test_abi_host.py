@@ -92,12 +92,13 @@ def test_workspace_bound_covers_the_plan(cl, cr, d):
     from mpsim_amd import _native as nat
     from mpsim_amd.batch import ws_bound
     for flags in (0, 1):
-        for method, fp32_sweeps in ((0, 0), (0, 12), (1, 0)):
+        for method, fp32_sweeps, pre in ((0, 0, 0), (0, 12, 0), (1, 0, 0), (0, 0, 2)):
             k = min(d * cl, d * cr)
             tab = _table([(cl, 8, cr, d, k, flags)] * 2)
             o = nat.SvdOpts(0, 0, 1, 0, 0.0)
             o.method = method
             o.fp32_sweeps = fp32_sweeps
+            o.precondition = pre
             rc, _, err = _plan(tab, o)
             assert rc == 0, err
             size = int(tab[1]["ws_off"] - tab[0]["ws_off"])
@@ -127,6 +128,41 @@ def test_plan_generic_qudit_theta_tiles():
     tiles = [7 * 2, 3 * 2, 2 * 1, 2 * 2]
     assert list(tab["tile_base"]) == [0, tiles[0], tiles[0] + tiles[1], sum(tiles[:3])]
     assert (tab[0]["C"], tab[0]["R"]) == (35, 350) and (tab[1]["C"], tab[1]["R"]) == (180, 27)
+
+
+def test_plan_qr_preconditioning_rule():
+    """precondition 1 marks (plan.method 3) the FP64 Gram problems whose
+    middle bond is at least the smaller outer bond; 2 every Gram problem up
+    to 512 padded rows and 4096 columns, plain splits included; the
+    one-sided rows problems and the mixed method are left alone."""
+    from mpsim_amd import _native as nat
+    rows = [(256, 256, 256, 2, 256, 0), (256, 128, 256, 2, 256, 0), (64, 300, 128, 2, 128, 1),
+            (257, 257, 257, 2, 514, 0), (32, 32, 32, 2, 32, 0), (300, 0, 200, 1, 200, 0)]
+    want = {0: [1, 1, 1, 1, 0, 1], 1: [3, 1, 3, 1, 0, 1], 2: [3, 3, 3, 1, 0, 3]}
+    for pre, methods in want.items():
+        tab = _table(rows)
+        tab[5]["gate"] = 0  # plain split
+        o = nat.SvdOpts(0, 0, 1, 0, 0.0)
+        o.precondition = pre
+        rc, _, err = _plan(tab, o)
+        assert rc == 0, err
+        assert list(tab["method"]) == methods, pre
+        if pre == 2:  # the preconditioned problem's region holds X0, panels and order
+            plain = _table(rows[:1] * 2)
+            _plan(plain)
+            pre_tab = _table(rows[:1] * 2)
+            _plan(pre_tab, o)
+            extra = int(pre_tab[1]["ws_off"]) - int(plain[1]["ws_off"])
+            assert extra >= 16 * 512 * 512 + 2 * 16 * 16 * 512
+    o = nat.SvdOpts(0, 0, 1, 0, 0.0)
+    o.precondition = 1
+    o.fp32_sweeps = 8
+    tab = _table(rows[:1])
+    assert _plan(tab, o)[0] == 0 and tab[0]["method"] == 2
+    o = nat.SvdOpts(0, 0, 1, 0, 0.0)
+    o.precondition = 3
+    rc, _, err = _plan(_table(rows[:1]), o)
+    assert rc == 1 and "precondition" in err
 
 
 def test_plan_rejects_bad_options():
