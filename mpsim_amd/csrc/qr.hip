@@ -146,9 +146,10 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
     __shared__ double2 Wp[8][16][17];  // per-wave partial V^H A
     __shared__ double2 Wt[16][17];     // V^H A
     __shared__ double2 Ws[16][17];     // T^H V^H A
-    __shared__ double2 Ts[16][17];     // T of the panel applied / factored
+    __shared__ double2 Ts[16][17];     // T of the panel applied
     __shared__ double2 Ys[16][17];     // Ys[j][k] = v_k^H v_j (k < j), for T
     __shared__ double taus[16];
+    __shared__ double2 scs[16], betas[16];
     __shared__ double red[8];
     __shared__ double2 redz[8][16];
     __shared__ double2 alpha_s;
@@ -249,6 +250,12 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
     if (factor) {
         rf = 16 * (panel + 1);
         ncols = min(16, qr_reflectors(p) - rf);
+        // Step j reflects column j (pivot row q = rf + j). The columns keep
+        // their unscaled x (below q) in registers until the end; sc_j, beta_j
+        // and tau_j go to LDS. Every lane reads w_j (x_j below q, 1 / sc_j at
+        // q, zero above) from the wave's LDS column, so v_j = sc_j w_j and
+        //   g'_k = w_j^H a_k,  a_k -= tau_j |sc_j|^2 g'_k w_j  (k > j),
+        //   v_m^H v_j = conj(sc_m) sc_j conj(g'_m)  (m < j: a_m holds x_m).
         for (int j = 0; j < ncols; ++j) {
             const int q = rf + j;
             // squared norm of column j below row q (every lane for its own
@@ -259,8 +266,8 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
-                    const double e = fma(a[i][r].x, a[i][r].x, a[i][r].y * a[i][r].y);
-                    s += row > q ? e : 0.0;
+                    const double2 x = zsel(row > q, a[i][r], make_double2(0.0, 0.0));
+                    s = fma(x.x, x.x, fma(x.y, x.y, s));
                     if (row == q && l16 == j) alpha_s = a[i][r];
                 }
             s += __shfl_xor(s, 16, 64);
@@ -270,7 +277,7 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
             const double s2 = ((red[0] + red[1]) + (red[2] + red[3])) + ((red[4] + red[5]) + (red[6] + red[7]));
             const double2 al = alpha_s;
             double tau = 0.0;
-            double2 beta = al, sc = make_double2(0.0, 0.0);
+            double2 beta = al, sc = make_double2(1.0, 0.0), isc = sc;
             if (s2 > 0.0) {  // two long operations deep: the square roots, then the reciprocals
                 const double a2 = al.x * al.x + al.y * al.y;
                 const double aa = sqrt(a2), xn = sqrt(a2 + s2);
@@ -278,38 +285,40 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
                 const double2 u = aa > 0.0 ? make_double2(al.x * ia, al.y * ia) : make_double2(1.0, 0.0);
                 beta = make_double2(-u.x * xn, -u.y * xn);
                 tau = 1.0 + aa * ix;
-                sc = make_double2(u.x * inv, -u.y * inv);  // conj(u) / (|x_q| + ||x||)
+                sc = make_double2(u.x * inv, -u.y * inv);          // conj(u) / (|x_q| + ||x||)
+                isc = make_double2(u.x * (aa + xn), u.y * (aa + xn));  // 1 / sc
             }
-            // column j: beta at q, v below (kept in place of the zeros). Lane
-            // j of each row also puts v (1 at q, 0 above) in the wave's LDS
-            // column, from which every lane of the row reads it: a wave owns
-            // its rows, so no barrier is needed (LDS operations of one wave
-            // complete in order).
-            const bool cj = l16 == j;
+            if (tid == 0) {
+                taus[j] = tau;
+                scs[j] = sc;
+                betas[j] = beta;
+            }
+            // w_j to the wave's LDS column (lane j of each row; the pivot
+            // row's entry after the others, by the same lane, in order)
+            if (l16 == j) {
 #pragma unroll
-            for (int i = 0; i < CH; ++i)
+                for (int i = 0; i < CH; ++i)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
-                    const double2 sv = zmul(a[i][r], sc);
-                    a[i][r] = zsel(cj, zsel(row == q, beta, zsel(row > q, sv, a[i][r])), a[i][r]);
-                    if (cj) vcol[w][16 * i + 4 * r + lq] = zsel(row > q, sv, make_double2(row == q ? 1.0 : 0.0, 0.0));
-                }
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
+                        vcol[w][16 * i + 4 * r + lq] = zsel(row > q, a[i][r], make_double2(0.0, 0.0));
+                    }
+                const int dq = q - rb, qw = (dq >> 4) & 7;
+                if (qw == w && ((dq & 3) == lq)) vcol[w][16 * (dq >> 7) + 4 * ((dq >> 2) & 3) + lq] = isc;
+            }
             __builtin_amdgcn_wave_barrier();
-            double2 vv[CH][4];
+            double2 ww[CH][4];
 #pragma unroll
             for (int i = 0; i < CH; ++i)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) vv[i][r] = vcol[w][16 * i + 4 * r + lq];
-            // g = v^H a for every column of the strip; columns k > j take
-            // a -= tau v g, columns k < j hold v_k and give conj(v_k^H v_j)
+                for (int r = 0; r < 4; ++r) ww[i][r] = vcol[w][16 * i + 4 * r + lq];
             double2 g = make_double2(0.0, 0.0);
 #pragma unroll
             for (int i = 0; i < CH; ++i)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    g.x += vv[i][r].x * a[i][r].x + vv[i][r].y * a[i][r].y;
-                    g.y += vv[i][r].x * a[i][r].y - vv[i][r].y * a[i][r].x;
+                    g.x = fma(ww[i][r].x, a[i][r].x, fma(ww[i][r].y, a[i][r].y, g.x));
+                    g.y = fma(ww[i][r].x, a[i][r].y, fma(-ww[i][r].y, a[i][r].x, g.y));
                 }
             g = zadd(g, shfl_xor_z(g, 16));
             g = zadd(g, shfl_xor_z(g, 32));
@@ -318,32 +327,38 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
             g = make_double2(0.0, 0.0);
 #pragma unroll
             for (int v = 0; v < 8; ++v) g = zadd(g, redz[v][l16]);
-            if (tid < 16 && tid < j) Ys[j][tid] = make_double2(g.x, -g.y);
-            if (tid == 0) taus[j] = tau;
-            const double2 tg = l16 > j ? zscale(g, tau) : make_double2(0.0, 0.0);
+            if (tid < 16 && tid < j)
+                Ys[j][tid] = zmul(zconj(scs[tid]), zmul(sc, zconj(g)));
+            const double cs = l16 > j ? tau * (sc.x * sc.x + sc.y * sc.y) : 0.0;
+            const double2 cg = zscale(g, cs);
 #pragma unroll
             for (int i = 0; i < CH; ++i)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    a[i][r].x -= vv[i][r].x * tg.x - vv[i][r].y * tg.y;
-                    a[i][r].y -= vv[i][r].x * tg.y + vv[i][r].y * tg.x;
+                    a[i][r].x = fma(-ww[i][r].x, cg.x, fma(ww[i][r].y, cg.y, a[i][r].x));
+                    a[i][r].y = fma(-ww[i][r].x, cg.y, fma(-ww[i][r].y, cg.x, a[i][r].y));
                 }
         }
         __syncthreads();
-        // T row k, one thread each (a compact loop: straight-line code run
-        // once per launch costs more in instruction fetch than it saves):
-        // T[k][k] = tau_k, T[k][j] = -tau_j sum_{m=k}^{j-1} T[k][m] Ys[j][m]
-        // for k < j < ncols, zero elsewhere
+        // T row k, one thread each, in registers with the loops unrolled (the
+        // fastest of the forms measured: a loop over LDS takes one round
+        // trip per term): T[k][k] = tau_k, T[k][j] = -tau_j sum_{m<j}
+        // T[k][m] Ys[j][m] for k < j < ncols, zero elsewhere
         if (tid < 16) {
             const int k = tid;
-            for (int j = 0; j < 16; ++j) Ts[k][j] = make_double2(j == k && k < ncols ? taus[k] : 0.0, 0.0);
-            for (int j = k + 1; j < ncols; ++j) {
+            double2 tr[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
                 double2 sum = make_double2(0.0, 0.0);
-                for (int m = k; m < j; ++m) sum = zadd(sum, zmul(Ts[k][m], Ys[j][m]));
-                Ts[k][j] = zscale(sum, -taus[j]);
+#pragma unroll
+                for (int m = 0; m < j; ++m) sum = zadd(sum, zmul(tr[m], Ys[j][m]));
+                const double tj = j < ncols ? taus[j] : 0.0;
+                tr[j] = zsel(j == k, make_double2(tj, 0.0),
+                             zsel(j > k && j < ncols, zscale(sum, -tj), make_double2(0.0, 0.0)));
             }
             double2 *T = qt_ptr(ws, p, panel + 1) + k * 16;
-            for (int j = 0; j < 16; ++j) T[j] = Ts[k][j];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) T[j] = tr[j];
         }
         // V rows rf .. C-1 of the panel
         double2 *V = qv_ptr(ws, p, panel + 1);
@@ -355,7 +370,7 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
                 if (row < rf || row >= C) continue;
                 const int q = rf + l16;
                 double2 v = make_double2(0.0, 0.0);
-                if (l16 < ncols) v = row > q ? a[i][r] : make_double2(row == q ? 1.0 : 0.0, 0.0);
+                if (l16 < ncols) v = zsel(row > q, zmul(a[i][r], scs[l16]), make_double2(row == q ? 1.0 : 0.0, 0.0));
                 V[(int64_t)row * 16 + l16] = v;
             }
     }
@@ -367,8 +382,11 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
         for (int r = 0; r < 4; ++r) {
             const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
             if (row >= C) continue;
-            const bool below = factor && l16 < ncols && row > rf + l16;
-            Z[(int64_t)row * LD + col] = below ? make_double2(0.0, 0.0) : a[i][r];
+            // reflected columns: beta on the diagonal, zero below
+            const bool refl = factor && l16 < ncols;
+            const double2 out = zsel(refl && row > rf + l16, make_double2(0.0, 0.0),
+                                     zsel(refl && row == rf + l16, betas[l16 & 15], a[i][r]));
+            Z[(int64_t)row * LD + col] = out;
         }
 }
 

@@ -23,7 +23,12 @@ Three stages, each timed on seeded synthetic two-site problems:
 
 Engine picks the stored values up as its defaults (tuned_config,
 tuned_engine). The library's compile-time tile shapes (theta tiles, 16-row
-Jacobi blocks, Gram tiles) are not tuned here.
+Jacobi blocks, Gram tiles) are not tuned here, and neither is the QR
+preconditioning of the Gram method (`precondition`, default 1: problems of
+saturated bonds): whether it saves sweeps depends on the spectrum of theta,
+which synthetic problems do not reproduce, so its default comes from whole
+circuit workloads (measured on the 64-qubit brickwork, the uneven batch and
+the served stream) and a stored value overrides it.
 """
 from __future__ import annotations
 
@@ -39,6 +44,7 @@ from .engine import SvdConfig
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNED = os.environ.get("MPSX_TUNED", os.path.join(HERE, "tuned.json"))
 
+DEFAULT_PRECONDITION = 1  # QR-precondition the Gram problems of saturated bonds
 DEFAULT_SPLIT_WAYS = 4   # the hardware queues a process gets
 DEFAULT_SPLIT_MIN = 8
 DEFAULT_SPLIT_ROWS = 128
@@ -58,7 +64,8 @@ def tuned_config(path: Optional[str] = None) -> SvdConfig:
     t = _load(path)
     return SvdConfig(block_switch=int(t.get("block_switch", 0)),
                      inner_sweeps=int(t.get("inner_sweeps", 0)),
-                     method=int(t.get("method", 0)))
+                     method=int(t.get("method", 0)),
+                     precondition=int(t.get("precondition", DEFAULT_PRECONDITION)))
 
 
 def tuned_engine(path: Optional[str] = None) -> Tuple[int, int, int]:
@@ -198,6 +205,7 @@ def tune(engine=None, chis: Sequence[int] = (8, 16, 32, 64, 128, 256), reps: int
     out["split_ways"] = split_ways
     out["split_rows"] = split_rows
     out["split_min"] = split_min
+    out["precondition"] = int(_load(path).get("precondition", DEFAULT_PRECONDITION))
     out["measurements"] = rows
     if path:
         with open(path, "w") as f:

@@ -3,6 +3,8 @@ CPU tensors: which pivot size, inner sweeps, SVD method and concurrency it
 picks, and that the stored result is what the engine reads back."""
 import types
 
+import json
+
 import numpy as np
 import torch
 
@@ -43,7 +45,17 @@ def test_tuner_selection_and_readback(tmp_path):
     assert stages == {"pivot", "method", "split", "split_min"}
     cfg = tune.tuned_config(path)
     assert (cfg.block_switch, cfg.inner_sweeps, cfg.method) == (32, 1, 0)
+    assert cfg.precondition == tune.DEFAULT_PRECONDITION == out["precondition"]
     assert tune.tuned_engine(path) == (3, 8, 128)
+    # the QR preconditioning is not tuned: a stored choice survives a re-tune
+    with open(path) as f:
+        stored = json.load(f)
+    stored["precondition"] = 0
+    with open(path, "w") as f:
+        json.dump(stored, f)
+    out = tune.tune(eng, chis=(8, 16), reps=1, path=path, conc_chis=(32,), layer=31,
+                    batch_sizes=(2,), timer=timer)
+    assert out["precondition"] == 0 and tune.tuned_config(path).precondition == 0
 
 
 def test_tuned_defaults_without_a_file(tmp_path):
@@ -51,6 +63,7 @@ def test_tuned_defaults_without_a_file(tmp_path):
     missing = str(tmp_path / "none.json")
     cfg = tune.tuned_config(missing)
     assert (cfg.block_switch, cfg.inner_sweeps, cfg.method) == (0, 0, 0)
+    assert cfg.precondition == tune.DEFAULT_PRECONDITION
     assert tune.tuned_engine(missing) == (tune.DEFAULT_SPLIT_WAYS, tune.DEFAULT_SPLIT_MIN,
                                           tune.DEFAULT_SPLIT_ROWS)
 
