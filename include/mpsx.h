@@ -183,7 +183,7 @@ typedef struct {
                            chi_r): theta built from an already truncated
                            bond);
                            2: every Gram problem (plain splits included).
-                           Only problems of at most 512 padded rows and
+                           Only problems of at most 1024 padded rows and
                            4096 columns; the others keep the plain Gram
                            method (plan.method 3 marks the preconditioned) */
 } mpsx_svd_opts;

@@ -147,8 +147,7 @@ MPSX_HD int64_t t_off(const mpsx_plan &p) {
 //   a 16-column panel, for even and odd panels) | T[2][16][16] (double2,
 //   the panel's triangular factor) | order[R] (int32, columns by
 //   decreasing norm)
-constexpr int QR_CHUNKS = 4;     // 16-row chunks a wave of the QR step holds
-constexpr int QR_MAX_CP = 128 * QR_CHUNKS;  // rows held by one workgroup's registers
+constexpr int QR_MAX_CP = 1024;  // rows one workgroup of the QR step holds (8 waves x 8 chunks of 16)
 constexpr int QR_MAX_R = 4096;   // a row staged in LDS
 MPSX_HD int64_t qx0_off(const mpsx_plan &p) { return t_off(p); }
 MPSX_HD int64_t qv_off(const mpsx_plan &p) { return qx0_off(p) + align_up(16LL * p.Cp * p.R, 256); }

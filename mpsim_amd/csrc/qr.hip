@@ -52,6 +52,13 @@ MPSX_DEV int *qorder_ptr(char *ws, const mpsx_plan &p) {
 MPSX_DEV double2 zsel(bool c, double2 x, double2 y) {
     return make_double2(c ? x.x : y.x, c ? x.y : y.y);
 }
+// With 8 chunks a wave (1024 rows) the strip takes 128 registers: keep the
+// scheduler from hoisting every chunk's loads ahead of the first (which
+// would take as many again and spill).
+template <int CH>
+MPSX_DEV void chunk_fence() {
+    if constexpr (CH > 4) __builtin_amdgcn_sched_barrier(0);
+}
 MPSX_DEV double2 shfl_xor_z(double2 v, int m) {
     return make_double2(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
 }
@@ -173,12 +180,14 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
 
     double2 a[CH][4];
 #pragma unroll
-    for (int i = 0; i < CH; ++i)
+    for (int i = 0; i < CH; ++i) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
             a[i][r] = (row < C && col_ok) ? Z[(int64_t)row * LD + col] : make_double2(0.0, 0.0);
         }
+        chunk_fence<CH>();
+    }
 
     if (update) {
         const double2 *V = qv_ptr(ws, p, panel);
@@ -198,6 +207,7 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
                 m2 = mfma16d(v.y, a[i][r].y, m2);
                 m3 = mfma16d(v.x - v.y, a[i][r].x + a[i][r].y, m3);
             }
+            chunk_fence<CH>();
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -243,6 +253,7 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
                 a[i][r].x -= n1[r] - n2[r];
                 a[i][r].y -= n3[r] - n1[r] - n2[r];
             }
+            chunk_fence<CH>();
         }
     }
 
@@ -258,18 +269,26 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
         //   v_m^H v_j = conj(sc_m) sc_j conj(g'_m)  (m < j: a_m holds x_m).
         for (int j = 0; j < ncols; ++j) {
             const int q = rf + j;
-            // squared norm of column j below row q (every lane for its own
-            // column; lane j's is used), and x_q
+            // Lane j of each row: x_j below q (zero elsewhere) into the
+            // wave's LDS column, its squared norm, and x_q. Row - q of element
+            // (i, r) is d0 + 128 i + 4 r; d0 passes through an empty asm
+            // statement, so that the compares are against constants and no
+            // predicate stays live from one pass to the next.
             double s = 0.0;
+            if (l16 == j) {
+                int d0 = rb + 16 * w + lq - q;
+                asm volatile("" : "+v"(d0));
 #pragma unroll
-            for (int i = 0; i < CH; ++i)
+                for (int i = 0; i < CH; ++i)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
-                    const double2 x = zsel(row > q, a[i][r], make_double2(0.0, 0.0));
-                    s = fma(x.x, x.x, fma(x.y, x.y, s));
-                    if (row == q && l16 == j) alpha_s = a[i][r];
-                }
+                    for (int r = 0; r < 4; ++r) {
+                        const int dr = d0 + 128 * i + 4 * r;
+                        const double2 x = zsel(dr > 0, a[i][r], make_double2(0.0, 0.0));
+                        vcol[w][16 * i + 4 * r + lq] = x;
+                        s = fma(x.x, x.x, fma(x.y, x.y, s));
+                        if (dr == 0) alpha_s = a[i][r];
+                    }
+            }
             s += __shfl_xor(s, 16, 64);
             s += __shfl_xor(s, 32, 64);
             if (lq == 0 && l16 == j) red[w] = s;
@@ -293,32 +312,36 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
                 scs[j] = sc;
                 betas[j] = beta;
             }
-            // w_j to the wave's LDS column (lane j of each row; the pivot
-            // row's entry after the others, by the same lane, in order)
+            // w_j = x_j with 1 / sc at the pivot row: that entry of the LDS
+            // column, written by the lane that wrote its zero (in order)
             if (l16 == j) {
-#pragma unroll
-                for (int i = 0; i < CH; ++i)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
-                        vcol[w][16 * i + 4 * r + lq] = zsel(row > q, a[i][r], make_double2(0.0, 0.0));
-                    }
                 const int dq = q - rb, qw = (dq >> 4) & 7;
                 if (qw == w && ((dq & 3) == lq)) vcol[w][16 * (dq >> 7) + 4 * ((dq >> 2) & 3) + lq] = isc;
             }
             __builtin_amdgcn_wave_barrier();
-            double2 ww[CH][4];
+            // w in registers for 4 chunks a wave; for 8 (1024 rows) the strip
+            // alone takes 128 registers, and w is read from LDS at each use
+            double2 ww[CH <= 4 ? CH : 1][4];
+            if constexpr (CH <= 4) {
 #pragma unroll
-            for (int i = 0; i < CH; ++i)
+                for (int i = 0; i < CH; ++i)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) ww[i][r] = vcol[w][16 * i + 4 * r + lq];
+                    for (int r = 0; r < 4; ++r) ww[i][r] = vcol[w][16 * i + 4 * r + lq];
+            }
+            auto wget = [&](int i, int r) -> double2 {
+                if constexpr (CH <= 4)
+                    return ww[i][r];
+                else
+                    return vcol[w][16 * i + 4 * r + lq];
+            };
             double2 g = make_double2(0.0, 0.0);
 #pragma unroll
             for (int i = 0; i < CH; ++i)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    g.x = fma(ww[i][r].x, a[i][r].x, fma(ww[i][r].y, a[i][r].y, g.x));
-                    g.y = fma(ww[i][r].x, a[i][r].y, fma(-ww[i][r].y, a[i][r].x, g.y));
+                    const double2 x = wget(i, r);
+                    g.x = fma(x.x, a[i][r].x, fma(x.y, a[i][r].y, g.x));
+                    g.y = fma(x.x, a[i][r].y, fma(-x.y, a[i][r].x, g.y));
                 }
             g = zadd(g, shfl_xor_z(g, 16));
             g = zadd(g, shfl_xor_z(g, 32));
@@ -335,11 +358,48 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
             for (int i = 0; i < CH; ++i)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    a[i][r].x = fma(-ww[i][r].x, cg.x, fma(ww[i][r].y, cg.y, a[i][r].x));
-                    a[i][r].y = fma(-ww[i][r].x, cg.y, fma(-ww[i][r].y, cg.x, a[i][r].y));
+                    const double2 x = wget(i, r);
+                    a[i][r].x = fma(-x.x, cg.x, fma(x.y, cg.y, a[i][r].x));
+                    a[i][r].y = fma(-x.x, cg.y, fma(-x.y, cg.x, a[i][r].y));
                 }
         }
         __syncthreads();
+        // V rows rf .. C-1 of the panel
+        double2 *V = qv_ptr(ws, p, panel + 1);
+#pragma unroll
+        for (int i = 0; i < CH; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
+                if (row < rf || row >= C) continue;
+                const int q = rf + l16;
+                double2 v = make_double2(0.0, 0.0);
+                if (l16 < ncols) v = zsel(row > q, zmul(a[i][r], scs[l16]), make_double2(row == q ? 1.0 : 0.0, 0.0));
+                V[(int64_t)row * 16 + l16] = v;
+            }
+    }
+
+    if (col_ok) {
+        // the row stride passes through an empty asm statement, so that the
+        // addresses are formed here rather than kept live from the loads
+        int ld = LD;
+        asm volatile("" : "+v"(ld));
+        double2 *zc = Z + col;
+#pragma unroll
+        for (int i = 0; i < CH; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
+                if (row >= C) continue;
+                // reflected columns: beta on the diagonal, zero below
+                const bool refl = factor && l16 < ncols;
+                const double2 out = zsel(refl && row > rf + l16, make_double2(0.0, 0.0),
+                                         zsel(refl && row == rf + l16, betas[l16 & 15], a[i][r]));
+                zc[(int64_t)row * ld] = out;
+            }
+    }
+    // T after the strip is stored, so that its registers are free
+    if (factor) {
         // T row k, one thread each, in registers with the loops unrolled (the
         // fastest of the forms measured: a loop over LDS takes one round
         // trip per term): T[k][k] = tau_k, T[k][j] = -tau_j sum_{m<j}
@@ -360,34 +420,7 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
 #pragma unroll
             for (int j = 0; j < 16; ++j) T[j] = tr[j];
         }
-        // V rows rf .. C-1 of the panel
-        double2 *V = qv_ptr(ws, p, panel + 1);
-#pragma unroll
-        for (int i = 0; i < CH; ++i)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
-                if (row < rf || row >= C) continue;
-                const int q = rf + l16;
-                double2 v = make_double2(0.0, 0.0);
-                if (l16 < ncols) v = zsel(row > q, zmul(a[i][r], scs[l16]), make_double2(row == q ? 1.0 : 0.0, 0.0));
-                V[(int64_t)row * 16 + l16] = v;
-            }
     }
-
-    if (!col_ok) return;
-#pragma unroll
-    for (int i = 0; i < CH; ++i)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = rb + 16 * (w + 8 * i) + 4 * r + lq;
-            if (row >= C) continue;
-            // reflected columns: beta on the diagonal, zero below
-            const bool refl = factor && l16 < ncols;
-            const double2 out = zsel(refl && row > rf + l16, make_double2(0.0, 0.0),
-                                     zsel(refl && row == rf + l16, betas[l16 & 15], a[i][r]));
-            Z[(int64_t)row * LD + col] = out;
-        }
 }
 
 hipError_t launch_qr_order(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws) {
@@ -403,8 +436,11 @@ hipError_t launch_qr_permute(hipStream_t s, const mpsx_two_site_op *d, int count
 
 hipError_t launch_qr_step(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int panel,
                           int strips, int max_c) {
-    if (max_c > 128 * QR_CHUNKS) return hipErrorInvalidValue;
+    if (max_c > QR_MAX_CP) return hipErrorInvalidValue;
     const dim3 grid((unsigned)count, (unsigned)strips);
-    k_qr_step<QR_CHUNKS><<<grid, 512, 0, s>>>(d, ws, panel);
+    if (max_c > 512)
+        k_qr_step<8><<<grid, 512, 0, s>>>(d, ws, panel);
+    else
+        k_qr_step<4><<<grid, 512, 0, s>>>(d, ws, panel);
     return hipGetLastError();
 }

@@ -133,12 +133,13 @@ def test_plan_generic_qudit_theta_tiles():
 def test_plan_qr_preconditioning_rule():
     """precondition 1 marks (plan.method 3) the FP64 Gram problems whose
     middle bond is at least the smaller outer bond; 2 every Gram problem up
-    to 512 padded rows and 4096 columns, plain splits included; the
+    to 1024 padded rows and 4096 columns, plain splits included; the
     one-sided rows problems and the mixed method are left alone."""
     from mpsim_amd import _native as nat
     rows = [(256, 256, 256, 2, 256, 0), (256, 128, 256, 2, 256, 0), (64, 300, 128, 2, 128, 1),
-            (257, 257, 257, 2, 514, 0), (32, 32, 32, 2, 32, 0), (300, 0, 200, 1, 200, 0)]
-    want = {0: [1, 1, 1, 1, 0, 1], 1: [3, 1, 3, 1, 0, 1], 2: [3, 3, 3, 1, 0, 3]}
+            (520, 520, 520, 2, 1040, 0), (32, 32, 32, 2, 32, 0), (300, 0, 200, 1, 200, 0),
+            (512, 512, 512, 2, 512, 0)]
+    want = {0: [1, 1, 1, 1, 0, 1, 1], 1: [3, 1, 3, 1, 0, 1, 3], 2: [3, 3, 3, 1, 0, 3, 3]}
     for pre, methods in want.items():
         tab = _table(rows)
         tab[5]["gate"] = 0  # plain split

@@ -36,12 +36,13 @@ def _check(theta, k, left, r, tag):
     return s
 
 
-# (chi_l, chi_m, chi_r, d, k, absorb_left): square 512 (the largest
-# preconditioned: 4 row chunks a wave), 500 rows (a partial last chunk), 600
-# rows (beyond the limit: the plain Gram method in the same call), wide,
-# tall with a partial last panel, rank-deficient (chi_m = 2), qutrits,
-# absorb-left orientations
+# (chi_l, chi_m, chi_r, d, k, absorb_left): square 512 (4 row chunks a
+# wave), 500 rows (a partial last chunk), 600 and 1024 rows (8 chunks a wave;
+# 1024 is the largest preconditioned), 1040 rows (beyond the limit: the plain
+# Gram method in the same call), wide, tall with a partial last panel,
+# rank-deficient (chi_m = 2), qutrits, absorb-left orientations
 SHAPES = [(256, 256, 256, 2, 256, False), (250, 250, 250, 2, 200, False), (300, 300, 300, 2, 300, False),
+          (512, 512, 512, 2, 512, False), (520, 520, 520, 2, 600, True),
           (60, 40, 200, 2, 100, False), (200, 40, 45, 2, 90, False),
           (100, 2, 100, 2, 50, False), (48, 48, 48, 3, 100, True),
           (45, 64, 200, 2, 90, True), (130, 130, 130, 2, 260, False)]
