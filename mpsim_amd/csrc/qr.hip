@@ -59,8 +59,188 @@ template <int CH>
 MPSX_DEV void chunk_fence() {
     if constexpr (CH > 4) __builtin_amdgcn_sched_barrier(0);
 }
+// Sum over the 64 lanes of a wave, the same in every lane (fixed order):
+// DPP swaps within quads, then the half-row and row mirrors, then two
+// cross-row exchanges.
+template <int CTRL>
+MPSX_DEV double dpp_d(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+MPSX_DEV double wave_sum(double x) {
+    x += dpp_d<0xB1>(x);   // quad_perm [1,0,3,2]
+    x += dpp_d<0x4E>(x);   // quad_perm [2,3,0,1]
+    x += dpp_d<0x141>(x);  // row_half_mirror
+    x += dpp_d<0x140>(x);  // row_mirror
+    x += __shfl_xor(x, 16, 64);
+    x += __shfl_xor(x, 32, 64);
+    return x;
+}
+// Element (rr, c) of the LDS copy of a 512 x 16 strip: rows of 16, the
+// column XOR-swizzled by the row so that a column read by consecutive rows
+// spreads over the banks.
+MPSX_DEV int sidx(int rr, int c) { return rr * 16 + (c ^ (rr & 15)); }
 MPSX_DEV double2 shfl_xor_z(double2 v, int m) {
     return make_double2(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+
+
+// T of the panel just factored, row k by thread k < 16, in registers with
+// the loops unrolled (the fastest of the forms measured: a loop over LDS
+// takes one round trip per term): T[k][k] = tau_k, T[k][j] = -tau_j
+// sum_{m<j} T[k][m] Ys[j][m] for k < j < ncols, zero elsewhere.
+MPSX_DEV void build_t(char *ws, const mpsx_plan &p, int panel, int ncols, const double2 (*Ys)[17],
+                      const double *taus) {
+    if (threadIdx.x >= 16) return;
+    const int k = threadIdx.x;
+    double2 tr[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        double2 sum = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int m = 0; m < j; ++m) sum = zadd(sum, zmul(tr[m], Ys[j][m]));
+        const double tj = j < ncols ? taus[j] : 0.0;
+        tr[j] = zsel(j == k, make_double2(tj, 0.0),
+                     zsel(j > k && j < ncols, zscale(sum, -tj), make_double2(0.0, 0.0)));
+    }
+    double2 *T = qt_ptr(ws, p, panel + 1) + k * 16;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) T[j] = tr[j];
+}
+
+// Factorisation of panel + 1 by the workgroup of strip 0 (the panel's own
+// columns, already updated by panel `panel` in `a`), for problems of at most
+// 512 rows. The strip goes through LDS into a column layout: wave w owns
+// columns 2w and 2w + 1 whole, lane l their rows rf + l + 64 t, so every
+// reduction over rows stays inside a wave (DPP, no barrier). Step j: the
+// owner of column j forms the reflector (norm below q, x_q, beta, tau, sc)
+// and publishes w_j (x_j below q, 1 / sc at q) in one of two LDS buffers
+// and its scalars; one barrier; every wave takes g'_k = w_j^H a_k for its
+// columns and updates those right of j (a_k -= tau |sc|^2 g'_k w_j), or
+// records v_k^H v_j for T (k < j). The buffers alternate, so the owner of
+// step j + 1 may publish while slower waves still read step j's.
+MPSX_DEV void factor_512(const mpsx_two_site_op *ops, char *ws, int panel, const double2 (&a)[4][4],
+                         double2 *big, double2 (*wbuf)[512], double2 (*Ys)[17], double *taus,
+                         double2 *scs, double2 *betas) {
+    const mpsx_plan p = ops[blockIdx.x].plan;
+    const int C = p.C, LD = p.LD, ncol = p.R + p.C;
+    const int rb = panel >= 0 ? 16 * panel : 0, rf = 16 * (panel + 1);
+    const int ncols = min(16, qr_reflectors(p) - rf);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, l16 = lane & 15, lq = lane >> 4;
+    double2 *Z = zq_ptr(ws, p);
+    __syncthreads();  // the update phase is done with the shared buffer
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) big[sidx(16 * (w + 8 * i) + 4 * r + lq, l16)] = a[i][r];
+    __syncthreads();
+    double2 x[2][8];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int rr = rf - rb + lane + 64 * t;
+            x[c][t] = rr < 512 ? big[sidx(rr, 2 * w + c)] : make_double2(0.0, 0.0);
+        }
+    for (int j = 0; j < ncols; ++j) {
+        const int q = rf + j, buf = j & 1;
+        if ((j >> 1) == w) {
+            const bool hi = (j & 1) != 0;
+            int d0 = rf + lane - q;  // row - q of element t is d0 + 64 t
+            asm volatile("" : "+v"(d0));
+            double s = 0.0;
+            double2 xq = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int dr = d0 + 64 * t;
+                const double2 xv = zsel(hi, x[1][t], x[0][t]);
+                const double2 xb = zsel(dr > 0, xv, make_double2(0.0, 0.0));
+                s = fma(xb.x, xb.x, fma(xb.y, xb.y, s));
+                xq = zsel(dr == 0, xv, xq);
+            }
+            const double s2 = wave_sum(s);
+            const double2 al = make_double2(wave_sum(xq.x), wave_sum(xq.y));
+            double tau = 0.0;
+            double2 beta = al, sc = make_double2(1.0, 0.0), isc = sc;
+            if (s2 > 0.0) {  // two long operations deep: the square roots, then the reciprocals
+                const double a2 = al.x * al.x + al.y * al.y;
+                const double aa = sqrt(a2), xn = sqrt(a2 + s2);
+                const double ia = 1.0 / aa, ix = 1.0 / xn, inv = 1.0 / (aa + xn);
+                const double2 u = aa > 0.0 ? make_double2(al.x * ia, al.y * ia) : make_double2(1.0, 0.0);
+                beta = make_double2(-u.x * xn, -u.y * xn);
+                tau = 1.0 + aa * ix;
+                sc = make_double2(u.x * inv, -u.y * inv);              // conj(u) / (|x_q| + ||x||)
+                isc = make_double2(u.x * (aa + xn), u.y * (aa + xn));  // 1 / sc
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int dr = d0 + 64 * t;
+                const double2 xv = zsel(hi, x[1][t], x[0][t]);
+                wbuf[buf][lane + 64 * t] = zsel(dr > 0, xv, zsel(dr == 0, isc, make_double2(0.0, 0.0)));
+            }
+            if (lane == 0) {
+                taus[j] = tau;
+                scs[j] = sc;
+                betas[j] = beta;
+            }
+        }
+        __syncthreads();
+        const double tau = taus[j];
+        const double2 sc = scs[j];
+        double2 wv[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) wv[t] = wbuf[buf][lane + 64 * t];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int k = 2 * w + c;
+            double2 g = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                g.x = fma(wv[t].x, x[c][t].x, fma(wv[t].y, x[c][t].y, g.x));
+                g.y = fma(wv[t].x, x[c][t].y, fma(-wv[t].y, x[c][t].x, g.y));
+            }
+            g = make_double2(wave_sum(g.x), wave_sum(g.y));
+            if (k > j) {
+                const double2 cg = zscale(g, tau * (sc.x * sc.x + sc.y * sc.y));
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    x[c][t].x = fma(-wv[t].x, cg.x, fma(wv[t].y, cg.y, x[c][t].x));
+                    x[c][t].y = fma(-wv[t].x, cg.y, fma(-wv[t].y, cg.x, x[c][t].y));
+                }
+            } else if (k < j && lane == 0) {
+                Ys[j][k] = zmul(zconj(scs[k]), zmul(sc, zconj(g)));
+            }
+        }
+    }
+    __syncthreads();
+    // reflected columns: beta on the diagonal and zero below in R, v = sc x
+    // below (1 at the diagonal) in V; the other columns as they are
+    double2 *V = qv_ptr(ws, p, panel + 1);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int k = 2 * w + c, qk = rf + k;
+        const bool refl = k < ncols;
+        const double2 sck = scs[k & 15], bk = betas[k & 15];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int row = rf + lane + 64 * t, rr = row - rb;
+            if (rr >= 512) continue;
+            const double2 xv = x[c][t];
+            big[sidx(rr, k)] = zsel(refl && row > qk, make_double2(0.0, 0.0), zsel(refl && row == qk, bk, xv));
+            if (row < C)
+                V[(int64_t)row * 16 + k] =
+                    refl ? zsel(row > qk, zmul(xv, sck), make_double2(row == qk ? 1.0 : 0.0, 0.0))
+                         : make_double2(0.0, 0.0);
+        }
+    }
+    __syncthreads();
+    // the strip to Z (rows rb .. C-1, 16 consecutive columns a row)
+    for (int e = tid; e < (C - rb) * 16; e += 512) {
+        const int rr = e >> 4, c = e & 15;
+        if (rf + c < ncol) Z[(int64_t)(rb + rr) * LD + rf + c] = big[sidx(rr, c)];
+    }
+    build_t(ws, p, panel, ncols, Ys, taus);
 }
 
 }  // namespace
@@ -150,17 +330,24 @@ __global__ void __launch_bounds__(256) k_qr_permute(const mpsx_two_site_op *ops,
 // lq, column l16) of lane (l16, lq), the matrix-core output layout.
 template <int CH>
 __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, char *ws, int panel) {
-    __shared__ double2 Wp[8][16][17];  // per-wave partial V^H A
-    __shared__ double2 Wt[16][17];     // V^H A
-    __shared__ double2 Ws[16][17];     // T^H V^H A
-    __shared__ double2 Ts[16][17];     // T of the panel applied
+    // The update phase's arrays and, for panels of at most 512 rows, the
+    // factorisation's LDS copy of the strip share one buffer.
+    constexpr int kUpd = 11 * 16 * 17;
+    constexpr int kStrip = CH == 4 ? 512 * 16 : 0;
+    __shared__ double2 big[kStrip > kUpd ? kStrip : kUpd];
+    auto Wp = reinterpret_cast<double2 (*)[16][17]>(big);             // per-wave partial V^H A
+    auto Wt = reinterpret_cast<double2 (*)[17]>(big + 8 * 16 * 17);   // V^H A
+    auto Ws = reinterpret_cast<double2 (*)[17]>(big + 9 * 16 * 17);   // T^H V^H A
+    auto Ts = reinterpret_cast<double2 (*)[17]>(big + 10 * 16 * 17);  // T of the panel applied
     __shared__ double2 Ys[16][17];     // Ys[j][k] = v_k^H v_j (k < j), for T
+    // 512-row panels: w_j published by the wave that owns column j, two buffers
+    __shared__ double2 wbuf[CH == 4 ? 2 : 1][CH == 4 ? 512 : 1];
     __shared__ double taus[16];
     __shared__ double2 scs[16], betas[16];
     __shared__ double red[8];
     __shared__ double2 redz[8][16];
     __shared__ double2 alpha_s;
-    __shared__ double2 vcol[8][16 * CH];  // per wave: v of the reflector, the wave's rows
+    __shared__ double2 vcol[8][CH == 4 ? 1 : 16 * CH];  // 1024-row panels: v, per wave for its rows
 
     const mpsx_plan p = ops[blockIdx.x].plan;
     if (p.method != 3) return;
@@ -254,6 +441,13 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
                 a[i][r].y -= n3[r] - n1[r] - n2[r];
             }
             chunk_fence<CH>();
+        }
+    }
+
+    if constexpr (CH == 4) {
+        if (factor) {
+            factor_512(ops, ws, panel, a, big, wbuf, Ys, taus, scs, betas);
+            return;
         }
     }
 
@@ -400,26 +594,7 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
     }
     // T after the strip is stored, so that its registers are free
     if (factor) {
-        // T row k, one thread each, in registers with the loops unrolled (the
-        // fastest of the forms measured: a loop over LDS takes one round
-        // trip per term): T[k][k] = tau_k, T[k][j] = -tau_j sum_{m<j}
-        // T[k][m] Ys[j][m] for k < j < ncols, zero elsewhere
-        if (tid < 16) {
-            const int k = tid;
-            double2 tr[16];
-#pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                double2 sum = make_double2(0.0, 0.0);
-#pragma unroll
-                for (int m = 0; m < j; ++m) sum = zadd(sum, zmul(tr[m], Ys[j][m]));
-                const double tj = j < ncols ? taus[j] : 0.0;
-                tr[j] = zsel(j == k, make_double2(tj, 0.0),
-                             zsel(j > k && j < ncols, zscale(sum, -tj), make_double2(0.0, 0.0)));
-            }
-            double2 *T = qt_ptr(ws, p, panel + 1) + k * 16;
-#pragma unroll
-            for (int j = 0; j < 16; ++j) T[j] = tr[j];
-        }
+        build_t(ws, p, panel, ncols, Ys, taus);
     }
 }
 
