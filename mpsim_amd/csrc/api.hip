@@ -421,6 +421,7 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
     bool has_b[17] = {false};  // one-sided problems by block width
     bool has_gram = false, has_mixed = false;
     int qr_c = 0, qr_rc = 0, qr_np = 0;  // QR-preconditioned problems: max C, R + C, panels
+    bool qr_small = false, qr_large = false;  // some of at most 512 rows, some above
     for (int i = 0; i < count; ++i) {
         const mpsx_two_site_op &o = h[i];
         const mpsx_plan &p = o.plan;
@@ -456,6 +457,7 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
             qr_c = std::max(qr_c, (int)p.C);
             qr_rc = std::max(qr_rc, (int)(p.R + p.C));
             qr_np = std::max(qr_np, qr_panels(p));
+            (p.C <= 512 ? qr_small : qr_large) = true;
         }
     }
     if (tiles > kGridLimit || pairs > kGridLimit || rows > kGridLimit || outs > kGridLimit)
@@ -491,7 +493,10 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
             const int strips = panel < 0 ? 1 : (qr_rc - 16 * (panel + 1) + 15) / 16;
             if (strips > 0)
                 HIP_TRY(timed("k_qr_step", stream,
-                              [&] { return launch_qr_step(stream, d, count, ws, panel, strips, qr_c); }),
+                              [&] {
+                                  return launch_qr_step(stream, d, count, ws, panel, strips, qr_c, qr_small,
+                                                        qr_large);
+                              }),
                         "k_qr_step");
         }
     }

@@ -351,6 +351,9 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
 
     const mpsx_plan p = ops[blockIdx.x].plan;
     if (p.method != 3) return;
+    // each problem always runs the same variant (4 chunks up to 512 rows, 8
+    // above), so its result does not depend on what else is in the batch
+    if ((CH == 4) != (p.C <= 512)) return;
     const int C = p.C, LD = p.LD, ncol = p.R + p.C;
     const int np = qr_panels(p);
     if (panel >= np) return;
@@ -610,12 +613,13 @@ hipError_t launch_qr_permute(hipStream_t s, const mpsx_two_site_op *d, int count
 }
 
 hipError_t launch_qr_step(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int panel,
-                          int strips, int max_c) {
+                          int strips, int max_c, bool small, bool large) {
     if (max_c > QR_MAX_CP) return hipErrorInvalidValue;
     const dim3 grid((unsigned)count, (unsigned)strips);
-    if (max_c > 512)
-        k_qr_step<8><<<grid, 512, 0, s>>>(d, ws, panel);
-    else
+    if (small) {
         k_qr_step<4><<<grid, 512, 0, s>>>(d, ws, panel);
+        if (hipError_t e = hipGetLastError()) return e;
+    }
+    if (large) k_qr_step<8><<<grid, 512, 0, s>>>(d, ws, panel);
     return hipGetLastError();
 }

@@ -133,3 +133,30 @@ def test_preconditioned_circuits_vs_oracle():
         print(c.name, "err", err, flush=True)
         assert err <= 1e-4, (c.name, err)
         assert abs(nr - nx) <= 1e-4 * nr
+
+
+def test_preconditioned_result_independent_of_batch():
+    """A problem's factors are bitwise the same alone and in a call with a
+    larger preconditioned problem (each runs the QR variant of its own size:
+    up to 512 rows, or above), so served and batched results do not depend
+    on the mix."""
+    import torch
+    from mpsim_amd import gates
+    from mpsim_amd.engine import Engine, SvdConfig, TwoSiteItem
+    eng = Engine.get()
+    rng = np.random.RandomState(91)
+
+    def item(cl, cm, cr, k):
+        A = ((rng.randn(cl, 2, cm) + 1j * rng.randn(cl, 2, cm)) / np.sqrt(4 * cm)).astype(np.complex64)
+        B = ((rng.randn(cm, 2, cr) + 1j * rng.randn(cm, 2, cr)) / np.sqrt(2)).astype(np.complex64)
+        return TwoSiteItem(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(),
+                           eng.gate(gates.haar_unitary(4, rng)), k, want_s=True)
+
+    small, large = item(200, 200, 200, 150), item(480, 480, 480, 300)
+    cfg = SvdConfig(block_switch=eng.svd.block_switch, inner_sweeps=eng.svd.inner_sweeps, precondition=1)
+    alone = eng.two_site([small], cfg)[0]
+    torch.cuda.synchronize()
+    mixed = eng.two_site([large, small], cfg)[1]
+    torch.cuda.synchronize()
+    for x, y in ((alone.a_out, mixed.a_out), (alone.b_out, mixed.b_out), (alone.s, mixed.s)):
+        assert np.array_equal(x.cpu().numpy(), y.cpu().numpy())
