@@ -112,9 +112,11 @@ typedef struct {
     int64_t out_base;  /* exclusive prefix of gather workgroups          */
     int32_t C;         /* rows orthogonalised (d*chi_l or d*chi_r)         */
     int32_t R;         /* row length (the other dimension)                */
-    int32_t Cp;        /* C padded to nb*b                                */
+    int32_t Cp;        /* C padded (to nb*b but for method 3 with C > R) */
     int32_t LD;        /* row stride of the working matrix (elements)     */
-    int32_t nb;        /* number of row blocks (even)                     */
+    int32_t nb;        /* number of row blocks the Jacobi pairs (even; for
+                          method 3 with C > R, R's blocks: the rows of the
+                          QR's R beyond R are zero)                      */
     int32_t b;         /* rows per block                                  */
     int32_t zbytes;    /* bytes per working-matrix element: 16 (complex
                           FP64 rotations) or 8 (complex FP32)             */

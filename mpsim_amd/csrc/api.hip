@@ -344,6 +344,14 @@ int plan_shape(const mpsx_two_site_op &o, int i, const Resolved &r, mpsx_plan *p
         (r.precondition == 2 ||
          (r.precondition == 1 && !split && o.chi_m >= std::min(o.chi_l, o.chi_r))))
         p->method = 3;
+    // Tall and preconditioned: R has no rows beyond R, so the Jacobi runs on
+    // R's 16-row blocks (when that still makes at least six), while Z, Y and
+    // the QR keep C's padded rows (Y is Cp wide, U has C rows)
+    if (p->method == 3 && p->C > p->R) {
+        int nb = (p->R + 15) / 16;
+        if (nb & 1) ++nb;
+        if (nb >= 6) p->nb = nb;
+    }
     return MPSX_OK;
 }
 

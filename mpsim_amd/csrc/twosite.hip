@@ -1026,8 +1026,11 @@ MPSX_DEV int rr_pivot(int item, int r, int n) {
 // operands staged through LDS), which re-reads the operands four times less
 // and wins on the large ones (theta of 3072 rows: H in 1.7 s instead of 4.6 s
 // over a 48-qutrit circuit).
+// The choice goes by the Jacobi's rows, 16 nb: for a tall QR-preconditioned
+// problem they are fewer than the working matrix's Cp rows, and the
+// launchers size their grids from nb too.
 constexpr int kTiledGemmRows = 512;
-MPSX_DEV bool tiled_gemm(const mpsx_plan &p) { return p.Cp > kTiledGemmRows; }
+MPSX_DEV bool tiled_gemm(const mpsx_plan &p) { return 16 * p.nb > kTiledGemmRows; }
 
 // One workgroup per upper 16x16 block tile (bi <= bj) of H = X X^H. x: the
 // problem's pivot slot j (nb/2 per problem), y in [0, nb]; the tile is

@@ -155,6 +155,16 @@ def test_plan_qr_preconditioning_rule():
             _plan(pre_tab, o)
             extra = int(pre_tab[1]["ws_off"]) - int(plain[1]["ws_off"])
             assert extra >= 16 * 512 * 512 + 2 * 16 * 16 * 512
+    # tall (C > R) and preconditioned: the Jacobi pairs R's blocks (at least
+    # six), Z and Y keep C's padded rows
+    for pre, want_nb in ((1, 6), (0, 26)):
+        tab = _table([(200, 200, 45, 2, 90, 0), (200, 200, 20, 2, 40, 0), (45, 64, 200, 2, 90, 1)])
+        o = nat.SvdOpts(0, 0, 1, 0, 0.0)
+        o.precondition = pre
+        assert _plan(tab, o)[0] == 0
+        assert (tab[0]["C"], tab[0]["R"], tab[0]["Cp"], tab[0]["nb"]) == (400, 90, 416, want_nb)
+        assert (tab[1]["Cp"], tab[1]["nb"]) == (416, 26)  # R's blocks would be too few
+        assert (tab[2]["C"], tab[2]["R"], tab[2]["Cp"], tab[2]["nb"]) == (400, 90, 416, want_nb)
     o = nat.SvdOpts(0, 0, 1, 0, 0.0)
     o.precondition = 1
     o.fp32_sweeps = 8
