@@ -112,11 +112,11 @@ def ws_bound(C: int, R: int) -> int:
     Cp x max(Cp, R) Gram, two sets of 32x32 pivot rotations (16 Cp x 64
     bytes) and flags, and the mixed Gram method's second Cp x max(Cp, R)
     region (the FP32 Gram, then X1 = Y X) or the QR-preconditioned one's
-    X0 (Cp x R), Householder panels (512 Cp bytes), triangular factors and
-    column order."""
+    X0 (Cp x R), ring of eight Householder panels (2048 Cp bytes),
+    triangular factors and column order."""
     cp = C + 31
-    return (16 * cp * (R + cp + 4) + 32 * cp * (cp if cp > R else R) + 16 * cp * 64 + 512 * cp
-            + 4 * R + 11008)
+    return (16 * cp * (R + cp + 4) + 32 * cp * (cp if cp > R else R) + 16 * cp * 64 + 2048 * cp
+            + 4 * R + 35840)
 
 
 def _ws_estimate(it: TwoSiteItem) -> int:

@@ -143,16 +143,21 @@ MPSX_HD int64_t t_off(const mpsx_plan &p) {
 // Problems of the QR-preconditioned Gram method (plan.method 3) add, at the
 // same offset as region T:
 //   X0[Cp][R] (double2, the unpermuted X, read by the final X <- Y X0;
-//   padded rows zero) | V[2][Cp][16] (double2, the Householder vectors of
-//   a 16-column panel, for even and odd panels) | T[2][16][16] (double2,
-//   the panel's triangular factor) | order[R] (int32, columns by
+//   padded rows zero) | V[QR_RING][Cp][16] (double2, the Householder vectors
+//   of a 16-column panel, panel k in slot k mod QR_RING) | T[QR_RING][16][16]
+//   (double2, the panels' triangular factors) | order[R] (int32, columns by
 //   decreasing norm)
 constexpr int QR_MAX_CP = 1024;  // rows one workgroup of the QR step holds (8 waves x 8 chunks of 16)
 constexpr int QR_MAX_R = 4096;   // a row staged in LDS
+// The columns of Y (Q^H being formed) take the panels' reflectors
+// QR_DEFER at a time (no later panel reads them); the ring holds those
+// panels and the one being factored meanwhile.
+constexpr int QR_DEFER = 4;
+constexpr int QR_RING = 8;
 MPSX_HD int64_t qx0_off(const mpsx_plan &p) { return t_off(p); }
 MPSX_HD int64_t qv_off(const mpsx_plan &p) { return qx0_off(p) + align_up(16LL * p.Cp * p.R, 256); }
-MPSX_HD int64_t qt_off(const mpsx_plan &p) { return qv_off(p) + 2 * 16LL * 16 * p.Cp; }
-MPSX_HD int64_t qorder_off(const mpsx_plan &p) { return qt_off(p) + 2 * 16LL * 256; }
+MPSX_HD int64_t qt_off(const mpsx_plan &p) { return qv_off(p) + QR_RING * 16LL * 16 * p.Cp; }
+MPSX_HD int64_t qorder_off(const mpsx_plan &p) { return qt_off(p) + QR_RING * 16LL * 256; }
 // Householder reflectors of the QR: one per column of X up to the last row.
 MPSX_HD int qr_reflectors(const mpsx_plan &p) { return p.C - 1 < p.R ? p.C - 1 : p.R; }
 MPSX_HD int qr_panels(const mpsx_plan &p) { return (qr_reflectors(p) + 15) / 16; }

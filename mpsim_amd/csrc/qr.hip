@@ -39,10 +39,10 @@ MPSX_DEV double2 *zq_ptr(char *ws, const mpsx_plan &p) {
     return reinterpret_cast<double2 *>(ws + z_off(p));
 }
 MPSX_DEV double2 *qv_ptr(char *ws, const mpsx_plan &p, int panel) {
-    return reinterpret_cast<double2 *>(ws + qv_off(p)) + (int64_t)(panel & 1) * 16 * p.Cp;
+    return reinterpret_cast<double2 *>(ws + qv_off(p)) + (int64_t)(panel % QR_RING) * 16 * p.Cp;
 }
 MPSX_DEV double2 *qt_ptr(char *ws, const mpsx_plan &p, int panel) {
-    return reinterpret_cast<double2 *>(ws + qt_off(p)) + (panel & 1) * 256;
+    return reinterpret_cast<double2 *>(ws + qt_off(p)) + (panel % QR_RING) * 256;
 }
 MPSX_DEV int *qorder_ptr(char *ws, const mpsx_plan &p) {
     return reinterpret_cast<int *>(ws + qorder_off(p));
@@ -362,7 +362,15 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
     const bool update = panel >= 0;
     const bool factor = blockIdx.y == 0 && panel + 1 < np;
     if (!update && !factor) return;
-    const int rb = update ? 16 * panel : 0;  // first row held
+    // A strip of Y columns only (c0 >= R) is read by no later panel: it takes
+    // the reflectors of QR_DEFER panels at a time, at every QR_DEFER-th
+    // launch and the last, from panel k0 on (one pass over it in HBM instead
+    // of one per panel). Strips with X columns, the factored one included,
+    // take every panel at once.
+    const bool ystrip = c0 >= p.R;
+    const int k0 = ystrip ? panel - panel % QR_DEFER : panel;
+    if (update && ystrip && panel % QR_DEFER != QR_DEFER - 1 && panel != np - 1) return;
+    const int rb = update ? 16 * k0 : 0;  // first row held
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, l16 = lane & 15, lq = lane >> 4;
     const int col = c0 + l16;
     const bool col_ok = col < ncol;
@@ -379,9 +387,10 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
         chunk_fence<CH>();
     }
 
-    if (update) {
-        const double2 *V = qv_ptr(ws, p, panel);
-        const double2 *T = qt_ptr(ws, p, panel);
+    for (int k = k0; update && k <= panel; ++k) {
+        const double2 *V = qv_ptr(ws, p, k);
+        const double2 *T = qt_ptr(ws, p, k);
+        const int vr0 = 16 * k;  // rows of panel k's reflectors (the ring slot's rows above are stale)
         // W = V^H A over this wave's rows: A operand conj(V[row][refl]),
         // B operand the strip; three real products per complex product
         mfma_d4 m1 = {0.0, 0.0, 0.0, 0.0}, m2 = m1, m3 = m1;
@@ -392,7 +401,7 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = base + 4 * r + lq;
-                const double2 v = row < C ? V[(int64_t)row * 16 + l16] : make_double2(0.0, 0.0);
+                const double2 v = (row >= vr0 && row < C) ? V[(int64_t)row * 16 + l16] : make_double2(0.0, 0.0);
                 m1 = mfma16d(v.x, a[i][r].x, m1);
                 m2 = mfma16d(v.y, a[i][r].y, m2);
                 m3 = mfma16d(v.x - v.y, a[i][r].x + a[i][r].y, m3);
@@ -432,7 +441,8 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
             const int vrow = base + l16;
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                const double2 v = vrow < C ? V[(int64_t)vrow * 16 + 4 * t + lq] : make_double2(0.0, 0.0);
+                const double2 v = (vrow >= vr0 && vrow < C) ? V[(int64_t)vrow * 16 + 4 * t + lq]
+                                                            : make_double2(0.0, 0.0);
                 const double2 x = Ws[4 * t + lq][l16];
                 n1 = mfma16d(v.x, x.x, n1);
                 n2 = mfma16d(v.y, x.y, n2);

@@ -58,7 +58,7 @@ def test_static_workspace_matches_caller_workspace():
     eng = Engine.get()
     rng = np.random.RandomState(71)
     # first batch: two theta 512 x 512 updates on the Gram path (12.5 MiB of
-    # workspace each, 16.8 MiB for the QR-preconditioned one of saturated
+    # workspace each, 17.5 MiB for the QR-preconditioned one of saturated
     # bonds), an absorb-left one, 16-row pivots, a 3-pivot Gram problem and a
     # plain split; second batch: smaller
     first = [(256, 256, 256, 256, False), (256, 128, 256, 200, True), (40, 32, 40, 64, False),
@@ -80,7 +80,7 @@ def test_static_workspace_rejects_a_batch_that_does_not_fit():
     from mpsim_amd.engine import Engine
     eng = Engine.get()
     rng = np.random.RandomState(72)
-    items = _items(eng, rng, [(256, 256, 256, 256, False)] * 6)  # 75 MiB (100.6 preconditioned)
+    items = _items(eng, rng, [(256, 256, 256, 256, False)] * 6)  # 75 MiB (105.2 preconditioned)
     lib = nat.load()
     cfg = eng.svd.to_c()
     outs, tab, dtab, _ = eng._prepare_two_site(items, cfg, workspace=False)
