@@ -53,8 +53,9 @@ MPSX_DEV double2 zsel(bool c, double2 x, double2 y) {
     return make_double2(c ? x.x : y.x, c ? x.y : y.y);
 }
 // With 8 chunks a wave (1024 rows) the strip takes 128 registers: keep the
-// scheduler from hoisting every chunk's loads ahead of the first (which
-// would take as many again and spill).
+// scheduler from hoisting every chunk's loads ahead of the first, which
+// would take as many registers again (the form still spills a few hundred
+// bytes a lane in its once-per-launch code).
 template <int CH>
 MPSX_DEV void chunk_fence() {
     if constexpr (CH > 4) __builtin_amdgcn_sched_barrier(0);
@@ -319,15 +320,17 @@ __global__ void __launch_bounds__(256) k_qr_permute(const mpsx_two_site_op *ops,
 }
 
 // One step of the blocked QR. x: problem; y: 16-column strip. Launch
-// `panel` >= 0 applies panel `panel`'s block reflector (V, T from buffer
-// panel & 1) to the rows panel*16 .. C-1 of the strip's columns
+// `panel` >= 0 applies panel `panel`'s block reflector (V, T from ring slot
+// panel mod QR_RING) to the rows panel*16 .. C-1 of the strip's columns
 // 16 (panel + 1 + y) .. +15 (the X columns right of the panel, then the Y
-// columns; every column of [X | Y] right of the panel up to R + C - 1);
-// strip 0 (the next panel's columns) then factors panel + 1 in registers
-// and writes its V and T to the other buffer. Launch -1 only factors panel
-// 0. Eight waves; wave w holds the 16-row chunks w, w + 8, ... of the
-// strip, chunk i as the four registers a[i][r] = (row 16 (w + 8 i) + 4 r +
-// lq, column l16) of lane (l16, lq), the matrix-core output layout.
+// columns; every column of [X | Y] right of the panel up to R + C - 1; the
+// strips of Y columns alone take QR_DEFER panels at a time, below);
+// strip 0 (the next panel's columns) then factors panel + 1 (factor_512 up
+// to 512 rows, else in the register layout) and writes its V and T to ring
+// slot panel + 1. Launch -1 only factors panel 0. Eight waves; wave w holds
+// the 16-row chunks w, w + 8, ... of the strip, chunk i as the four
+// registers a[i][r] = (row 16 (w + 8 i) + 4 r + lq, column l16) of lane
+// (l16, lq), the matrix-core output layout.
 template <int CH>
 __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, char *ws, int panel) {
     // The update phase's arrays and, for panels of at most 512 rows, the
