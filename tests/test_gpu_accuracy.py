@@ -69,7 +69,8 @@ def test_gram_and_row_methods_agree():
     products and the kept singular values agree to the precision of their
     complex64 / float32 outputs (1e-6 of ||theta||, of sigma_max), the FP64
     discarded weights to 1e-10 of ||theta||^2, and both match LAPACK on theta
-    in FP64."""
+    in FP64. The QR-preconditioned Gram method (precondition 2: every Gram
+    problem) meets the same bars against the row iteration."""
     import torch
     from mpsim_amd import gates
     from mpsim_amd.engine import Engine, SvdConfig, TwoSiteItem
@@ -93,13 +94,15 @@ def test_gram_and_row_methods_agree():
                        np.tensordot(A.astype(np.complex128), B.astype(np.complex128), axes=([2], [0])))
         thetas.append(th.reshape(2 * cl, 2 * cr))
     out = {}
-    for method in (0, 1):
-        res = eng.two_site(items, SvdConfig(block_switch=16, method=method))
+    for key, method, pre in (("gram", 0, 0), ("rows", 1, 0), ("qr", 0, 2)):
+        res = eng.two_site(items, SvdConfig(block_switch=16, method=method, precondition=pre))
         torch.cuda.synchronize()
-        out[method] = [(r.a_out.cpu().numpy().astype(np.complex128),
-                        r.b_out.cpu().numpy().astype(np.complex128),
-                        r.s.cpu().numpy().astype(np.float64), float(r.discarded.cpu())) for r in res]
-    for (cl, cm, cr, k, left), th, g, o in zip(shapes, thetas, out[0], out[1]):
+        out[key] = [(r.a_out.cpu().numpy().astype(np.complex128),
+                     r.b_out.cpu().numpy().astype(np.complex128),
+                     r.s.cpu().numpy().astype(np.float64), float(r.discarded.cpu())) for r in res]
+    cases = [(sh, th, g, o) for gk in ("gram", "qr")
+             for sh, th, g, o in zip(shapes, thetas, out[gk], out["rows"])]
+    for (cl, cm, cr, k, left), th, g, o in cases:
         M, N = th.shape
         rec_g = g[0].reshape(M, k) @ g[1].reshape(k, N)
         rec_o = o[0].reshape(M, k) @ o[1].reshape(k, N)

@@ -17,6 +17,9 @@ Bounds (|s_gpu - s_ref| per kept value; ||theta||_F = sqrt(sum sigma^2)):
       accuracy. The truncation is unaffected (discarded weight to 1e-10
       ||theta||^2), and so is the state: the factors are X~ = Y X with Y
       unitary to FP64.
+  QR-preconditioned Gram method (precondition 2): the same bound. Its H =
+      R R^H = Q^H (X X^H) Q carries the same rounding, and X~ = (W Q^H) X0
+      with W Q^H unitary to FP64.
 """
 import numpy as np
 import pytest
@@ -43,17 +46,17 @@ def test_graded_spectrum_per_value(n):
     fro = float(np.linalg.norm(t64))
     eng = Engine.get()
     dev = torch.from_numpy(theta).cuda()
-    for method, floor in ((1, 0.0), (0, 2e-6)):
+    for method, pre, floor in ((1, 0, 0.0), (0, 0, 2e-6), (0, 2, 2e-6)):
         for k in (n, n // 2):
             r = eng.two_site([SplitItem(dev, k, want_s=True)],
-                             dataclasses.replace(eng.svd, method=method))[0]
+                             dataclasses.replace(eng.svd, method=method, precondition=pre))[0]
             s = r.s.cpu().numpy().astype(np.float64)
             err = np.abs(s - s_ref[:k])
             big = s_ref[:k] >= 1e-2 * fro
-            print(f"n={n} method={method} k={k}: max rel err {np.max(err / s_ref[:k]):.2e} "
+            print(f"n={n} method={method} precondition={pre} k={k}: max rel err {np.max(err / s_ref[:k]):.2e} "
                   f"(sigma >= 1e-2 |theta|: {np.max(err[big] / s_ref[:k][big]):.2e}), "
                   f"max abs err / |theta| {np.max(err) / fro:.2e}")
-            assert np.all(err <= 1e-4 * s_ref[:k] + floor * fro), (method, k)
+            assert np.all(err <= 1e-4 * s_ref[:k] + floor * fro), (method, pre, k)
             disc = float(r.discarded.cpu())
             assert abs(disc - np.sum(s_ref[k:] ** 2)) <= 1e-10 * fro ** 2
             a = r.a_out.cpu().numpy().astype(np.complex128).reshape(n, k)
