@@ -641,7 +641,7 @@ def main():
         "rounds": jr_n,
         "avg_round_us": round(1e3 * jr_ms / max(jr_n, 1), 3),
         "avg_launch_us": {k: round(1e3 * kern[k][0] / max(kern[k][1], 1), 3)
-                          for k in kset if k in kern},
+                          for k in busy_kernels(svd) if k in kern},
         "executed_mfma_frac_counters": (
             round(mc["executed_flop_per_round"] * jr_n / (jr_busy * 1e-3) / 1e12 / peak, 4)
             if (mc and not svd.fp32 and jr_busy > 0 and mc.get("kernel") == dom) else None),
