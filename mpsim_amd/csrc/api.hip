@@ -428,7 +428,7 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
     bool has_d[THETA_GEN_DMAX + 1] = {};  // index 1: plain splits
     bool has_b[17] = {false};  // one-sided problems by block width
     bool has_gram = false, has_mixed = false;
-    int qr_c = 0, qr_rc = 0, qr_np = 0;  // QR-preconditioned problems: max C, R + C, panels
+    int qr_c = 0, qr_r = 0, qr_rc = 0, qr_np = 0;  // QR-preconditioned problems: max C, R, R + C, panels
     bool qr_small = false, qr_large = false;  // some of at most 512 rows, some above
     for (int i = 0; i < count; ++i) {
         const mpsx_two_site_op &o = h[i];
@@ -463,6 +463,7 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
         }
         if (p.nb > 0 && p.method == 3) {
             qr_c = std::max(qr_c, (int)p.C);
+            qr_r = std::max(qr_r, (int)p.R);
             qr_rc = std::max(qr_rc, (int)(p.R + p.C));
             qr_np = std::max(qr_np, qr_panels(p));
             (p.C <= 512 ? qr_small : qr_large) = true;
@@ -494,7 +495,7 @@ int two_site_run(hipStream_t stream, const mpsx_two_site_op *h, const mpsx_two_s
     }
     if (qr_c > 0) {
         // [X P | I] -> [R | Q^H] for the preconditioned problems, X0 saved
-        HIP_TRY(timed("k_qr_order", stream, [&] { return launch_qr_order(stream, d, count, ws); }), "k_qr_order");
+        HIP_TRY(timed("k_qr_order", stream, [&] { return launch_qr_order(stream, d, count, ws, qr_c, qr_r); }), "k_qr_order");
         HIP_TRY(timed("k_qr_permute", stream, [&] { return launch_qr_permute(stream, d, count, ws, rows); }),
                 "k_qr_permute");
         for (int panel = -1; panel < qr_np; ++panel) {

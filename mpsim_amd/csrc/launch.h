@@ -52,7 +52,8 @@ hipError_t launch_hxgemm(hipStream_t s, const mpsx_two_site_op *d, const int *ow
 // QR preconditioning (plan.method 3): column order, X0 save and permute,
 // and step `panel` (-1 factors panel 0) over `strips` 16-column strips, for
 // the problems of at most 512 rows (`small`) and those above (`large`).
-hipError_t launch_qr_order(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws);
+hipError_t launch_qr_order(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int max_c,
+                           int max_r);
 hipError_t launch_qr_permute(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws,
                              int64_t rows);
 hipError_t launch_qr_step(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int panel,

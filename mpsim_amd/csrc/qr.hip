@@ -246,40 +246,63 @@ MPSX_DEV void factor_512(const mpsx_two_site_op *ops, char *ws, int panel, const
 
 }  // namespace
 
-// Column order of X, one workgroup per problem: squared column norms in
-// FP64, then the rank of each column under (norm descending, index
-// ascending); order[rank] = column. A NaN norm ranks last, so the order is
-// a permutation whatever the data.
-__global__ void __launch_bounds__(1024) k_qr_order(const mpsx_two_site_op *ops, char *ws) {
-    __shared__ double nrm[QR_MAX_R];
-    __shared__ double part[4][256];
+// Squared column norms of X in parts: workgroup (problem, y, z) sums the
+// rows 32 y .. 32 y + 31 of the columns 256 z .. 256 z + 255 in row order
+// into part[y][column] (FP64), so that many workgroups share the problem's
+// read of X. part lies in the V ring, unused until the first panel is
+// factored.
+namespace {
+constexpr int QR_NORM_ROWS = 32;
+constexpr int QR_NORM_COLS = 256;
+// Cp is a multiple of 32, so a problem has at most Cp / 32 parts
+static_assert((QR_MAX_CP / QR_NORM_ROWS) * QR_MAX_R * 8 <= QR_RING * 16 * 16 * QR_MAX_CP,
+              "the column norm parts fit the V ring");
+MPSX_DEV double *qnorm_ptr(char *ws, const mpsx_plan &p) {
+    return reinterpret_cast<double *>(ws + qv_off(p));
+}
+}  // namespace
+
+__global__ void __launch_bounds__(QR_NORM_COLS) k_qr_colnorm(const mpsx_two_site_op *ops, char *ws) {
     const mpsx_plan p = ops[blockIdx.x].plan;
     if (p.method != 3) return;
-    const int C = p.C, R = p.R, LD = p.LD;
-    const double2 *Z = zq_ptr(ws, p);
-    const int tid = threadIdx.x, g = tid >> 8, t = tid & 255;
-    for (int c0 = 0; c0 < R; c0 += 256) {
-        const int c = c0 + t;
-        double s = 0.0;
-        if (c < R) {
+    const int C = p.C, R = p.R;
+    const int r0 = (int)blockIdx.y * QR_NORM_ROWS, c = (int)blockIdx.z * QR_NORM_COLS + (int)threadIdx.x;
+    if (r0 >= C || c >= R) return;
+    const int r1 = r0 + QR_NORM_ROWS < C ? r0 + QR_NORM_ROWS : C;
+    const double2 *z = zq_ptr(ws, p) + c;
+    double s = 0.0;
 #pragma unroll 8
-            for (int r = g; r < C; r += 4) {
-                const double2 v = Z[(int64_t)r * LD + c];
-                s = fma(v.x, v.x, fma(v.y, v.y, s));
-            }
-        }
-        part[g][t] = s;
-        __syncthreads();
-        if (g == 0 && c < R) {
-            const double v = (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
-            nrm[c] = v == v ? v : -1.0;
-        }
-        __syncthreads();
+    for (int r = r0; r < r1; ++r) {
+        const double2 v = z[(int64_t)r * p.LD];
+        s = fma(v.x, v.x, fma(v.y, v.y, s));
     }
+    qnorm_ptr(ws, p)[(int64_t)blockIdx.y * R + c] = s;
+}
+
+// Column order of X, one workgroup per problem: the squared column norms
+// (the parts summed in row order), then the rank of each column under (norm
+// descending, index ascending); order[rank] = column. A NaN norm ranks
+// last, so the order is a permutation whatever the data.
+__global__ void __launch_bounds__(1024) k_qr_order(const mpsx_two_site_op *ops, char *ws) {
+    __shared__ double nrm[QR_MAX_R];
+    const mpsx_plan p = ops[blockIdx.x].plan;
+    if (p.method != 3) return;
+    const int R = p.R, parts = (p.C + QR_NORM_ROWS - 1) / QR_NORM_ROWS;
+    const int tid = threadIdx.x;
+    const double *part = qnorm_ptr(ws, p);
+    for (int c = tid; c < R; c += 1024) {
+        double v = 0.0;
+        for (int y = 0; y < parts; ++y) v += part[(int64_t)y * R + c];
+        nrm[c] = v == v ? v : -1.0;
+    }
+    __syncthreads();
     int *order = qorder_ptr(ws, p);
     for (int c = tid; c < R; c += 1024) {
         const double v = nrm[c];
         int rank = 0;
+        // unrolled so that the broadcast LDS reads overlap (one at a time,
+        // their latency sets the kernel's time)
+#pragma unroll 16
         for (int o = 0; o < R; ++o) {
             const double w = nrm[o];
             rank += (w > v) || (w == v && o < c);
@@ -614,7 +637,13 @@ __global__ void __launch_bounds__(512) k_qr_step(const mpsx_two_site_op *ops, ch
     }
 }
 
-hipError_t launch_qr_order(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws) {
+hipError_t launch_qr_order(hipStream_t s, const mpsx_two_site_op *d, int count, char *ws, int max_c,
+                           int max_r) {
+    if (max_c > QR_MAX_CP || max_r > QR_MAX_R) return hipErrorInvalidValue;
+    const dim3 parts((unsigned)count, (unsigned)((max_c + QR_NORM_ROWS - 1) / QR_NORM_ROWS),
+                     (unsigned)((max_r + QR_NORM_COLS - 1) / QR_NORM_COLS));
+    k_qr_colnorm<<<parts, QR_NORM_COLS, 0, s>>>(d, ws);
+    if (hipError_t e = hipGetLastError()) return e;
     k_qr_order<<<count, 1024, 0, s>>>(d, ws);
     return hipGetLastError();
 }

@@ -1069,7 +1069,13 @@ k_hgram(const mpsx_two_site_op *ops, const int *owner, char *ws) {
     const int LD = STAGE == 0 ? p.LD : p.R, R = p.R;
     const int64_t ra = (int64_t)(bi * 16 + l16) * LD, rb = (int64_t)(bj * 16 + l16) * LD;
     mfma_d4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = p1, p3 = p1;
-    for (int c0 = wave * 16; c0 < R; c0 += 64) {
+    // A QR-preconditioned problem's X is R, upper trapezoidal (row i is zero
+    // left of column i): the 16-column blocks left of row block bj hold zeros
+    // in the tile's second operand and add exact zeros, so they are skipped
+    // (the sums are bitwise those of the full loop).
+    int cs = wave * 16;
+    if (STAGE == 0 && p.method == 3 && 16 * bj >= cs + 16) cs += 64 * ((16 * bj - 16 - cs) / 64 + 1);
+    for (int c0 = cs; c0 < R; c0 += 64) {
         const int cq = c0 + 4 * lq;
         double2 va[4], vb[4];
 #pragma unroll
