@@ -176,6 +176,38 @@ def test_plan_qr_preconditioning_rule():
     assert rc == 1 and "precondition" in err
 
 
+def test_qr_plans_meet_the_kernels_assumptions():
+    """Every QR-preconditioned plan (random shapes, both orientations, up to
+    the size limits) has what the QR kernels take for granted: padded rows a
+    multiple of 32 and at most 1024, at most 4096 columns, an even block
+    count, and column-norm parts (one FP64 value per column and 32 rows)
+    that fit the V ring of 8 x 16 x 16 bytes per padded row."""
+    from mpsim_amd import _native as nat
+    rng = np.random.RandomState(12)
+    rows = []
+    for _ in range(400):
+        d = int(rng.choice([2, 3, 4]))
+        cl, cr = (int(x) for x in rng.randint(1, 1200 // d, size=2))
+        cm = int(rng.randint(1, 600))
+        rows.append((cl, cm, cr, d, int(rng.randint(1, 1 + d * min(cl, cr))), int(rng.randint(0, 2))))
+    rows += [(512, 512, 512, 2, 512, 0), (100, 100, 2048, 2, 200, 0), (2048, 100, 100, 2, 200, 1),
+             (340, 340, 341, 3, 1020, 0)]
+    o = nat.SvdOpts(0, 0, 1, 0, 0.0)
+    o.precondition = 2
+    seen = 0
+    for lo in range(0, len(rows), 50):
+        tab = _table(rows[lo:lo + 50])
+        rc, _, err = _plan(tab, o)
+        assert rc == 0, err
+        for r in tab[tab["method"] == 3]:
+            seen += 1
+            C, R, Cp, nb = int(r["C"]), int(r["R"]), int(r["Cp"]), int(r["nb"])
+            assert Cp % 32 == 0 and C <= Cp <= 1024 and R <= 4096, (C, R, Cp)
+            assert nb % 2 == 0 and nb >= 6 and 16 * nb <= Cp, (C, R, Cp, nb)
+            assert -(-C // 32) * R * 8 <= 8 * 16 * 16 * Cp, (C, R, Cp)
+    assert seen > 100
+
+
 def test_plan_rejects_bad_options():
     from mpsim_amd import _native as nat
     rc, _, err = _plan(_table([(4, 4, 4, 2, 8, 0)]), nat.SvdOpts(5, 0, 1, 0, 0.0))
