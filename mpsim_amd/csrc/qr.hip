@@ -246,11 +246,6 @@ MPSX_DEV void factor_512(const mpsx_two_site_op *ops, char *ws, int panel, const
 
 }  // namespace
 
-// Squared column norms of X in parts: workgroup (problem, y, z) sums the
-// rows 32 y .. 32 y + 31 of the columns 256 z .. 256 z + 255 in row order
-// into part[y][column] (FP64), so that many workgroups share the problem's
-// read of X. part lies in the V ring, unused until the first panel is
-// factored.
 namespace {
 constexpr int QR_NORM_ROWS = 32;
 constexpr int QR_NORM_COLS = 256;
@@ -262,6 +257,11 @@ MPSX_DEV double *qnorm_ptr(char *ws, const mpsx_plan &p) {
 }
 }  // namespace
 
+// Squared column norms of X in parts: workgroup (problem, y, z) sums the
+// rows 32 y .. 32 y + 31 of the columns 256 z .. 256 z + 255 in row order
+// into part[y][column] (FP64), so that many workgroups share the problem's
+// read of X. part lies in the V ring, unused until the first panel is
+// factored.
 __global__ void __launch_bounds__(QR_NORM_COLS) k_qr_colnorm(const mpsx_two_site_op *ops, char *ws) {
     const mpsx_plan p = ops[blockIdx.x].plan;
     if (p.method != 3) return;
@@ -300,8 +300,7 @@ __global__ void __launch_bounds__(1024) k_qr_order(const mpsx_two_site_op *ops, 
     for (int c = tid; c < R; c += 1024) {
         const double v = nrm[c];
         int rank = 0;
-        // unrolled so that the broadcast LDS reads overlap (one at a time,
-        // their latency sets the kernel's time)
+        // unrolled so that the broadcast LDS reads overlap
 #pragma unroll 16
         for (int o = 0; o < R; ++o) {
             const double w = nrm[o];
