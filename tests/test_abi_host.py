@@ -105,6 +105,29 @@ def test_workspace_bound_covers_the_plan(cl, cr, d):
             assert ws_bound(int(tab[0]["C"]), int(tab[0]["R"])) >= size, (flags, method, size)
 
 
+def test_workspace_bound_covers_random_plans():
+    """batch.ws_bound against the planned region on 300 random shapes (both
+    orientations, qudits 2-4, up to 1200 rows), for each SVD method and
+    preconditioning setting."""
+    from mpsim_amd import _native as nat
+    from mpsim_amd.batch import ws_bound
+    rng = np.random.RandomState(21)
+    shapes = []
+    for _ in range(300):
+        d = int(rng.choice([2, 3, 4]))
+        cl, cr = (int(x) for x in rng.randint(1, 1200 // d, size=2))
+        shapes.append((cl, int(rng.randint(1, 600)), cr, d, int(rng.randint(0, 2))))
+    for method, fp32_sweeps, pre in ((0, 0, 0), (0, 12, 0), (1, 0, 0), (0, 0, 1), (0, 0, 2)):
+        o = nat.SvdOpts(0, 0, 1, 0, 0.0)
+        o.method, o.fp32_sweeps, o.precondition = method, fp32_sweeps, pre
+        for cl, cm, cr, d, flags in shapes:
+            tab = _table([(cl, cm, cr, d, min(d * cl, d * cr), flags)] * 2)
+            rc, _, err = _plan(tab, o)
+            assert rc == 0, err
+            size = int(tab[1]["ws_off"] - tab[0]["ws_off"])
+            assert ws_bound(int(tab[0]["C"]), int(tab[0]["R"])) >= size, (cl, cm, cr, d, flags, method, pre)
+
+
 @pytest.mark.parametrize("row,msg", [
     ((4, 4, 4, 2, 9, 0), "k=9"),
     ((4, 4, 4, 17, 2, 0), "d=17"),
