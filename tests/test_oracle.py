@@ -352,3 +352,15 @@ def test_rdm_planted_mistake_is_caught():
     assert np.allclose(good, ref, atol=1e-12)
     assert not np.allclose(good.T, ref, atol=1e-3)
     assert not np.allclose(m.reduced_density_matrix([0, 2]), ref, atol=1e-3)
+
+
+def test_cpu_baseline_runs_on_a_small_sample():
+    """The timed CPU baseline (the oracle at the cap shape; the 16-qubit
+    plain workload) on small samples: the records carry their counts and
+    positive rates (a 6-qubit depth-2 plain run has 3 + 2 CNOTs)."""
+    from oracle.cpu_baseline import blas_threads, time_plain16, time_updates
+    r = time_updates(updates=2, n=16, cap=32, threads=1)
+    assert r["updates"] == 2 and r["seconds"] > 0 and r["updates_per_s"] > 0
+    r = time_plain16(n=6, depth=2)
+    assert r["updates"] == 5 and r["seconds"] > 0
+    assert blas_threads() >= 1
